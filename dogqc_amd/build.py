@@ -1,0 +1,52 @@
+"""Ahead-of-time build: the runtime library and every named plan module.
+
+``buildAll()`` compiles ``csrc/dqc_runtime.hip`` into ``libdqcrt.so`` and
+every plan of ``planlib.PLANS`` into ``_plans/`` (in parallel), always from
+source, for gfx950.
+"""
+import os
+import subprocess
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+from dogqc_amd import plancache
+
+PKG = plancache.PKG
+RUNTIME_SRC = os.path.join(PKG, "csrc", "dqc_runtime.hip")
+RUNTIME_SO = os.path.join(PKG, "libdqcrt.so")
+
+
+def buildRuntime():
+    tmp = RUNTIME_SO + ".tmp"
+    cmd = [plancache.HIPCC] + plancache.HIPCC_FLAGS + ["-I", plancache.INCLUDE, RUNTIME_SRC, "-o", tmp, "-ldl"]
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("building libdqcrt.so failed:\n" + proc.stderr[-6000:])
+    os.replace(tmp, RUNTIME_SO)
+    return RUNTIME_SO
+
+
+def buildPlans(names=None, jobs=None, force=False):
+    from dogqc_amd import planlib
+    names = list(planlib.PLANS) if names is None else names
+    comps = [planlib.translate(n) for n in names]
+    if force:
+        for c, n in zip(comps, names):
+            p = plancache.modulePath(c.source, n)
+            if os.path.exists(p):
+                os.unlink(p)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        paths = list(ex.map(lambda cn: cn[0].compile(cn[1]), zip(comps, names)))
+    return dict(zip(names, paths))
+
+
+def buildAll(verbose=True):
+    t0 = time.time()
+    buildRuntime()
+    paths = buildPlans(force=True)
+    if verbose:
+        print("built libdqcrt.so and %d plan modules in %.1f s" % (len(paths), time.time() - t0),
+              file=sys.stderr)
+    return paths

@@ -1,0 +1,471 @@
+// Device runtime for generated DogQC pipelines on MI355X (gfx950, CDNA4).
+//
+// Replaces the reference's CUDA runtime headers dogqc/include/hashing.cuh and
+// dogqc/include/util.cuh with primitives designed for 64-lane wavefronts:
+//   * 64-bit ballots and mbcnt lane prefixes instead of 32-lane masks
+//     (util.cuh:3, codegen.py:82-114);
+//   * keyed hash tables with lock-free CAS on a 64-bit tag word instead of a
+//     CAS on a hash followed by an unsynchronised payload store
+//     (hashing.cuh:81-119) or a OnceLock spin (hashing.cuh:206-272);
+//   * a multimap (count -> exclusive scan -> scatter) with a deterministic
+//     device-wide scan instead of per-warp atomic range allocation
+//     (hashing.cuh:139-199);
+//   * per-workgroup LDS aggregation tables (per-lane private accumulators or
+//     LDS atomics) flushed once per workgroup into the global table, instead
+//     of one global atomic per row and aggregate (cudaTranslator.py:466-493);
+//   * SQL string helpers (util.cuh:61-199).
+//
+// Tag words: 0 = empty slot, 1 = slot being claimed (hashed mode only),
+// otherwise bit 63 is set.  In "exact" mode the remaining 63 bits hold the
+// packed key itself, so a tag match is a key match and no payload compare or
+// lock is needed.  In "hashed" mode they hold a 63-bit hash and the keys live
+// in payload columns.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef DQC_DEVICE_INLINE
+#define DQC_DEVICE_INLINE __device__ __forceinline__
+#endif
+
+namespace dqc {
+
+constexpr int WAVE = 64;
+constexpr uint64_t TAG_EMPTY = 0ull;
+constexpr uint64_t TAG_LOCKED = 1ull;
+constexpr uint64_t TAG_TOP = 1ull << 63;
+
+// status word bits (first word of every plan workspace)
+constexpr uint32_t ST_HT_OVERFLOW = 1u << 0;    // some keyed table ran full
+constexpr uint32_t ST_MM_OVERFLOW = 1u << 1;    // multimap entry array too small
+constexpr uint32_t ST_RESULT_OVERFLOW = 1u << 2;
+constexpr uint32_t ST_STRING_LITERAL = 1u << 3;
+
+// ---------------------------------------------------------------- wave64
+
+DQC_DEVICE_INLINE uint32_t lane_id() { return __lane_id(); }
+
+DQC_DEVICE_INLINE uint64_t ballot(bool p) { return (uint64_t)__ballot((int)p); }
+
+// number of set bits of m in lanes strictly below this lane
+DQC_DEVICE_INLINE uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+DQC_DEVICE_INLINE uint32_t popc64(uint64_t m) { return (uint32_t)__popcll(m); }
+
+DQC_DEVICE_INLINE int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+
+// read `v` from lane `src` (any trivially copyable type, moved as 32-bit words)
+template <typename T>
+DQC_DEVICE_INLINE T shfl(T v, int src) {
+    constexpr int N = (int)((sizeof(T) + 3) / 4);
+    uint32_t w[N];
+    for (int i = 0; i < N; ++i) w[i] = 0u;
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = (uint32_t)__shfl((int)w[i], src, WAVE);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
+// a value known to be equal in all lanes, moved to scalar registers
+DQC_DEVICE_INLINE uint64_t uniform64(uint64_t v) {
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// inclusive prefix sum over the 64 lanes (all lanes must participate)
+DQC_DEVICE_INLINE uint64_t wave_incl_scan(uint64_t v) {
+    const int l = (int)lane_id();
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint64_t o = __shfl_up(v, (unsigned)d, WAVE);
+        if (l >= d) v += o;
+    }
+    return v;
+}
+
+template <typename T>
+DQC_DEVICE_INLINE T wave_sum(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+template <typename T>
+DQC_DEVICE_INLINE T wave_min(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) { T o = __shfl_xor(v, d, WAVE); v = o < v ? o : v; }
+    return v;
+}
+
+template <typename T>
+DQC_DEVICE_INLINE T wave_max(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) { T o = __shfl_xor(v, d, WAVE); v = o > v ? o : v; }
+    return v;
+}
+
+// ---------------------------------------------------------------- hashing
+// 64-bit finaliser of MurmurHash3 (Austin Appleby, public domain).
+DQC_DEVICE_INLINE uint64_t mix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// chain one more key column into a running hash
+DQC_DEVICE_INLINE uint64_t hash_combine(uint64_t h, uint64_t k) {
+    return mix64(h ^ (k + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)));
+}
+
+// map a hash onto [0, cap) with a multiply-shift on its upper half
+// (D. Lemire, "A fast alternative to the modulo reduction", 2016)
+DQC_DEVICE_INLINE uint32_t slot_of(uint64_t h, uint32_t cap) {
+    return (uint32_t)(((h >> 32) * (uint64_t)cap) >> 32);
+}
+
+DQC_DEVICE_INLINE uint32_t next_slot(uint32_t s, uint32_t cap) { return (s + 1u == cap) ? 0u : s + 1u; }
+
+// ------------------------------------------------------------ status word
+DQC_DEVICE_INLINE void raise_status(uint32_t* status, uint32_t bit) { atomicOr(status, bit); }
+
+// -------------------------------------------------------- keyed hash tables
+// Global tables: tags[cap] (uint64) + structure-of-arrays payload columns.
+
+// Insert a tag; duplicates of an existing tag take the next free slot (the
+// unique-join build of the reference also stores duplicate keys,
+// hashing.cuh:81-100).  Returns the slot, or -1 when the table is full.
+DQC_DEVICE_INLINE int64_t ht_insert(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
+    uint32_t s = slot_of(h, cap);
+    for (uint32_t n = 0; n < cap; ++n) {
+        uint64_t prev = atomicCAS((unsigned long long*)&tags[s], (unsigned long long)TAG_EMPTY,
+                                  (unsigned long long)tag);
+        if (prev == TAG_EMPTY) return (int64_t)s;
+        s = next_slot(s, cap);
+    }
+    return -1;
+}
+
+// Find the next slot holding `tag`, starting at *cursor (initialise the
+// cursor with probe_start).  Returns -1 at the first empty slot or after a
+// full cycle.  Used by join probes, which run after the build kernel ended.
+DQC_DEVICE_INLINE uint32_t probe_start(uint64_t h, uint32_t cap) { return slot_of(h, cap); }
+
+DQC_DEVICE_INLINE int64_t ht_probe(const uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t& cursor,
+                                   uint32_t& visited) {
+    while (visited < cap) {
+        uint32_t s = cursor;
+        uint64_t t = tags[s];
+        cursor = next_slot(s, cap);
+        ++visited;
+        if (t == tag) return (int64_t)s;
+        if (t == TAG_EMPTY) { visited = cap; return -1; }
+    }
+    return -1;
+}
+
+// Group slot in exact mode: the tag is the key, so find-or-claim is one CAS
+// per probed slot with no lock.  Returns -1 when the table is full.
+DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
+    uint32_t s = slot_of(h, cap);
+    for (uint32_t n = 0; n < cap; ++n) {
+        uint64_t t = __hip_atomic_load(&tags[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == tag) return (int64_t)s;
+        if (t == TAG_EMPTY) {
+            uint64_t prev = atomicCAS((unsigned long long*)&tags[s], (unsigned long long)TAG_EMPTY,
+                                      (unsigned long long)tag);
+            if (prev == TAG_EMPTY || prev == tag) return (int64_t)s;
+        }
+        s = next_slot(s, cap);
+    }
+    return -1;
+}
+
+// Group slot in hashed mode.  The claiming lane CASes EMPTY->LOCKED, writes
+// the key columns through `write_keys(slot)`, then publishes the tag with a
+// release store; other lanes acquire-load the tag and compare keys through
+// `keys_equal(slot)`.  The claim branch precedes the re-check in program
+// order, so lanes of one wavefront waiting on a slot claimed by a sibling
+// lane always observe the publication on their next loop trip.
+template <typename WriteKeys, typename KeysEqual>
+DQC_DEVICE_INLINE int64_t agg_slot_hashed(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h,
+                                          WriteKeys write_keys, KeysEqual keys_equal) {
+    uint32_t s = slot_of(h, cap);
+    uint32_t n = 0;
+    while (n < cap) {
+        uint64_t t = __hip_atomic_load(&tags[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == TAG_EMPTY) {
+            uint64_t expected = TAG_EMPTY;
+            if (__hip_atomic_compare_exchange_strong(&tags[s], &expected, TAG_LOCKED, __ATOMIC_ACQ_REL,
+                                                     __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+                write_keys(s);
+                __hip_atomic_store(&tags[s], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                return (int64_t)s;
+            }
+            continue;  // lost the race: re-read this slot
+        }
+        if (t == TAG_LOCKED) continue;  // being published by another lane
+        if (t == tag && keys_equal(s)) return (int64_t)s;
+        s = next_slot(s, cap);
+        ++n;
+    }
+    return -1;
+}
+
+// LDS variant of agg_slot_exact (workgroup-private table of `cap` slots).
+DQC_DEVICE_INLINE int lds_slot_exact(uint64_t* ltags, uint32_t cap, uint64_t tag, uint64_t h) {
+    uint32_t s = slot_of(h, cap);
+    for (uint32_t n = 0; n < cap; ++n) {
+        uint64_t t = __hip_atomic_load(&ltags[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (t == tag) return (int)s;
+        if (t == TAG_EMPTY) {
+            uint64_t expected = TAG_EMPTY;
+            if (__hip_atomic_compare_exchange_strong(&ltags[s], &expected, tag, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                return (int)s;
+            if (expected == tag) return (int)s;
+        }
+        s = next_slot(s, cap);
+    }
+    return -1;
+}
+
+// ------------------------------------------------------------ atomics
+DQC_DEVICE_INLINE void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+DQC_DEVICE_INLINE void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+DQC_DEVICE_INLINE void atomic_add(int32_t* p, int32_t v) { atomicAdd(p, v); }
+DQC_DEVICE_INLINE void atomic_add(uint32_t* p, uint32_t v) { atomicAdd(p, v); }
+DQC_DEVICE_INLINE void atomic_add(int64_t* p, int64_t v) {
+    atomicAdd((unsigned long long*)p, (unsigned long long)v);
+}
+
+template <typename T>
+DQC_DEVICE_INLINE void atomic_min_fp(T* p, T v) {
+    T cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (v < cur) {
+        if (__hip_atomic_compare_exchange_weak(p, &cur, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT))
+            break;
+    }
+}
+
+template <typename T>
+DQC_DEVICE_INLINE void atomic_max_fp(T* p, T v) {
+    T cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (v > cur) {
+        if (__hip_atomic_compare_exchange_weak(p, &cur, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT))
+            break;
+    }
+}
+
+DQC_DEVICE_INLINE void atomic_min(double* p, double v) { atomic_min_fp(p, v); }
+DQC_DEVICE_INLINE void atomic_max(double* p, double v) { atomic_max_fp(p, v); }
+DQC_DEVICE_INLINE void atomic_min(float* p, float v) { atomic_min_fp(p, v); }
+DQC_DEVICE_INLINE void atomic_max(float* p, float v) { atomic_max_fp(p, v); }
+DQC_DEVICE_INLINE void atomic_min(int32_t* p, int32_t v) { atomicMin(p, v); }
+DQC_DEVICE_INLINE void atomic_max(int32_t* p, int32_t v) { atomicMax(p, v); }
+DQC_DEVICE_INLINE void atomic_min(uint32_t* p, uint32_t v) { atomicMin(p, v); }
+DQC_DEVICE_INLINE void atomic_max(uint32_t* p, uint32_t v) { atomicMax(p, v); }
+DQC_DEVICE_INLINE void atomic_min(int64_t* p, int64_t v) { atomicMin((long long*)p, (long long)v); }
+DQC_DEVICE_INLINE void atomic_max(int64_t* p, int64_t v) { atomicMax((long long*)p, (long long)v); }
+
+// LDS atomics (workgroup scope); ds_add_f64 is native on gfx950
+DQC_DEVICE_INLINE void lds_add(double* p, double v) { atomicAdd(p, v); }
+DQC_DEVICE_INLINE void lds_add(float* p, float v) { atomicAdd(p, v); }
+DQC_DEVICE_INLINE void lds_add(int32_t* p, int32_t v) { atomicAdd(p, v); }
+DQC_DEVICE_INLINE void lds_add(int64_t* p, int64_t v) { atomicAdd((unsigned long long*)p, (unsigned long long)v); }
+template <typename T>
+DQC_DEVICE_INLINE void lds_min(T* p, T v) {
+    T cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (v < cur) {
+        if (__hip_atomic_compare_exchange_weak(p, &cur, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP))
+            break;
+    }
+}
+template <typename T>
+DQC_DEVICE_INLINE void lds_max(T* p, T v) {
+    T cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (v > cur) {
+        if (__hip_atomic_compare_exchange_weak(p, &cur, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP))
+            break;
+    }
+}
+
+// ------------------------------------------------------------- strings
+// String values in registers are [start, end) byte ranges (util.h:11-14).
+struct str_t {
+    const char* start;
+    const char* end;
+};
+
+// result string columns store byte offsets into the source char heap
+struct str_offs {
+    uint64_t start;
+    uint64_t end;
+};
+
+DQC_DEVICE_INLINE str_t string_scan(const uint64_t* offs, const char* chars, int64_t ix) {
+    str_t s;
+    s.start = chars + offs[ix];
+    s.end = chars + offs[ix + 1];
+    return s;
+}
+
+DQC_DEVICE_INLINE str_offs to_string_offset(const char* chars, str_t s) {
+    str_offs o;
+    o.start = (uint64_t)(s.start - chars);
+    o.end = (uint64_t)(s.end - chars);
+    return o;
+}
+
+DQC_DEVICE_INLINE bool string_equals(str_t a, str_t b) {
+    const int64_t n = a.end - a.start;
+    if (n != b.end - b.start) return false;
+    for (int64_t i = 0; i < n; ++i)
+        if (a.start[i] != b.start[i]) return false;
+    return true;
+}
+
+DQC_DEVICE_INLINE uint64_t string_hash(str_t s) {
+    uint64_t h = 0x84222325cbf29ce4ull;
+    for (const char* c = s.start; c < s.end; ++c) h = (h ^ (uint8_t)*c) * 0x100000001b3ull;  // FNV-1a
+    return mix64(h);
+}
+
+// SQL SUBSTRING(s FROM from FOR len), 1-based, clipped to the string
+// (semantics of util.cuh:132-142)
+DQC_DEVICE_INLINE str_t string_substring(str_t s, int from, int len) {
+    str_t r;
+    r.start = s.start + (from - 1);
+    r.end = r.start + len;
+    if (r.start < s.start) r.start = s.start;
+    if (r.end > s.end) r.end = s.end;
+    if (r.start > r.end) r.start = r.end;
+    return r;
+}
+
+// SQL LIKE with '%' (any run) and '_' (any one byte): iterative wildcard
+// matching that backtracks to the most recent '%' (textbook greedy method).
+DQC_DEVICE_INLINE bool string_like(str_t s, str_t p) {
+    const char* si = s.start;
+    const char* pi = p.start;
+    const char* star = nullptr;
+    const char* mark = nullptr;
+    while (si < s.end) {
+        if (pi < p.end && (*pi == '_' || *pi == *si)) {
+            ++si;
+            ++pi;
+        } else if (pi < p.end && *pi == '%') {
+            star = pi++;
+            mark = si;
+        } else if (star) {
+            pi = star + 1;
+            si = ++mark;
+        } else {
+            return false;
+        }
+    }
+    while (pi < p.end && *pi == '%') ++pi;
+    return pi == p.end;
+}
+
+// ---------------------------------------------------- multimap scan kernels
+// Bucket directory of a multimap: dir[2b] = offset of bucket b's entries,
+// dir[2b+1] = its count.  The count pass increments dir[2b+1]; these three
+// kernels turn counts into offsets by an exclusive scan (per-block totals, a
+// scan of the block totals, and a per-block scan that adds the block's
+// base) and clear the counts, which the insert pass reuses as cursors (a
+// count ends equal to its bucket's size again once every entry is in).
+constexpr int SCAN_BS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_BS * SCAN_ITEMS;
+
+__device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t* sh, uint64_t& total) {
+    const int t = threadIdx.x;
+    const int w = t / WAVE;
+    uint64_t incl = wave_incl_scan(v);
+    if ((int)lane_id() == WAVE - 1) sh[w] = incl;
+    __syncthreads();
+    if (t == 0) {
+        uint64_t run = 0;
+        for (int i = 0; i < SCAN_BS / WAVE; ++i) {
+            uint64_t x = sh[i];
+            sh[i] = run;
+            run += x;
+        }
+        sh[SCAN_BS / WAVE] = run;
+    }
+    __syncthreads();
+    uint64_t r = sh[w] + incl - v;
+    total = sh[SCAN_BS / WAVE];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(SCAN_BS) dqc_mm_scan_reduce(const uint32_t* dir, uint64_t n,
+                                                               uint64_t* block_sums) {
+    __shared__ uint64_t sh[SCAN_BS / WAVE + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t ix = base + (uint64_t)i * SCAN_BS + threadIdx.x;
+        if (ix < n) acc += dir[2 * ix + 1];
+    }
+    uint64_t total;
+    block_excl_scan(acc, sh, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// single workgroup: exclusive scan of the block totals, writes the grand total
+__global__ void __launch_bounds__(SCAN_BS) dqc_mm_scan_blocks(uint64_t* block_sums, uint64_t nblocks,
+                                                               uint64_t* total_out) {
+    __shared__ uint64_t sh[SCAN_BS / WAVE + 1];
+    uint64_t carry = 0;
+    for (uint64_t base = 0; base < nblocks; base += SCAN_BS) {
+        uint64_t ix = base + threadIdx.x;
+        uint64_t v = ix < nblocks ? block_sums[ix] : 0;
+        uint64_t total;
+        uint64_t ex = block_excl_scan(v, sh, total);
+        if (ix < nblocks) block_sums[ix] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BS) dqc_mm_scan_apply(uint32_t* dir, uint64_t n,
+                                                              const uint64_t* block_sums) {
+    __shared__ uint64_t sh[SCAN_BS / WAVE + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t ix = base + i;
+        v[i] = ix < n ? dir[2 * ix + 1] : 0u;
+        acc += v[i];
+    }
+    uint64_t total;
+    uint64_t run = block_excl_scan(acc, sh, total) + block_sums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t ix = base + i;
+        if (ix < n) {
+            dir[2 * ix] = (uint32_t)run;
+            dir[2 * ix + 1] = 0u;
+        }
+        run += v[i];
+    }
+}
+
+}  // namespace dqc

@@ -1,0 +1,367 @@
+"""Hash equi-join translation (build and probe) for gfx950.
+
+Same decisions as the reference's EquiJoinTranslator (dogqc/hashJoins.py):
+the left child builds, the right child probes; ``multimatch`` (cfg, default
+True) selects a multimap, otherwise a unique table; INNER/SEMI/ANTI join
+types; ``EquiJoinTranslator.usePushDownJoin`` (hashJoins.py:14) selects
+push-down parallelism for multimatch probes.  The data structures and the
+push-down are designed for 64-lane wavefronts:
+
+  * unique table: 64-bit tags claimed by CAS.  When the key columns pack into
+    63 bits the tag is the key itself ("exact" mode): the probe compares one
+    word and the build-side key values are taken from the probe-side keys, so
+    they are not stored.  Otherwise the tag is a hash and keys are stored and
+    compared.  Payload columns are structure-of-arrays and loaded lazily.
+  * multimap: bucket directory of (offset, count) pairs, count pass, scan,
+    insert pass (hashing.cuh:122-199), entries stored as columns.
+  * push-down: the wavefront scans the match-list lengths of its 64 lanes
+    and walks the concatenated matches 64 at a time; each lane finds the probe
+    tuple owning its match by binary search over the scan in LDS and takes
+    that tuple's registers by a lane shuffle.  Every lane does useful work in
+    every trip except the last, however skewed the lists (the reference packs
+    whole lists into 32 lanes with a leader loop, hashJoins.py:599-681).
+
+Extension over the reference: additional non-equality join conditions are
+evaluated for INNER joins too (the reference evaluates them only for
+SEMI/ANTI, hashJoins.py:203-204, 264-265).
+"""
+from collections import OrderedDict
+
+from dogqc_amd.relationalAlgebra import Join
+from dogqc_amd.translatorBase import BinaryTranslator
+from dogqc_amd.types import Type
+from dogqc_amd.hiplang import CType, PACK_BITS
+
+
+class EquiJoinTranslator(BinaryTranslator):
+
+    usePushDownJoin = False
+
+    def __init__(self, algExpr, leftChild, rightChild):
+        BinaryTranslator.__init__(self, algExpr, leftChild, rightChild)
+        self.consumeCall = 0
+
+    def produce(self, ctxt):
+        self.leftChild.produce(ctxt)
+        self.rightChild.produce(ctxt)
+
+    def consume(self, ctxt):
+        self.consumeCall += 1
+        if self.consumeCall % 2 == 1:
+            self.consumeBuild(ctxt)
+        else:
+            self.consumeProbe(ctxt)
+
+    # ----------------------------------------------------------- planning
+    def _plan(self, ctxt):
+        a = self.algExpr
+        self.buildKeys = list(a.buildKeyAttributes.values())
+        self.probeKeys = list(a.probeKeyAttributes.values())
+        leftOut = self.leftChild.algExpr.outRelation
+        if a.joinType in (Join.INNER, Join.OUTER):
+            above = [att for id, att in a.outRelation.items() if id in leftOut]
+        else:
+            above = []
+        condBuild = list(getattr(a, "conditionBuildAttributes", {}).values())
+        self.exact = (all(PACK_BITS.get(b.dataType) for b in self.buildKeys)
+                      and sum(PACK_BITS[b.dataType] for b in self.buildKeys) <= 63
+                      and all(b.dataType == p.dataType for b, p in zip(self.buildKeys, self.probeKeys)))
+        keyIds = set(b.id for b in self.buildKeys)
+        stored = OrderedDict()
+        if a.multimatch or not self.exact:
+            for b in self.buildKeys:
+                stored[b.id] = b
+        for att in above + condBuild:
+            if self.exact and not a.multimatch and att.id in keyIds:
+                continue
+            stored[att.id] = att
+        self.stored = list(stored.values())
+        self.keyReconstruct = [b for b in self.buildKeys if b.id not in stored and
+                               any(b.id == x.id for x in above + condBuild)]
+
+    def _hashCode(self, pipe, keys, tagVar, hashVar):
+        if self.exact:
+            parts = ["dqc::TAG_TOP"]
+            shift = 0
+            for k in keys:
+                w = PACK_BITS[k.dataType]
+                mask = "0xffffffffull" if w == 32 else "0xffull"
+                parts.append("(((uint64_t)(%s) & %s) << %d)" % (pipe.access(k), mask, shift))
+                shift += w
+            pipe.emit("const uint64_t %s = %s;" % (tagVar, " | ".join(parts)))
+            pipe.emit("const uint64_t %s = dqc::mix64(%s);" % (hashVar, tagVar))
+        else:
+            from dogqc_amd.hipTranslator import _keyBits
+            pipe.emit("uint64_t %s = 0;" % hashVar)
+            for k in keys:
+                pipe.emit("%s = dqc::hash_combine(%s, %s);" % (hashVar, hashVar, _keyBits(k, pipe.access(k))))
+            pipe.emit("const uint64_t %s = dqc::TAG_TOP | (%s >> 1);" % (tagVar, hashVar))
+
+    # ----------------------------------------------------------- build
+    def consumeBuild(self, ctxt):
+        a = self.algExpr
+        pipe = ctxt.pipe
+        prog = ctxt.program
+        self._plan(ctxt)
+        J = a.opId
+        pipe.emit("// hash join build (opId %d)" % J)
+        pipe.need(self.buildKeys + self.stored)
+        est = prog.estimate(self.leftChild.algExpr)
+        kind = "multi" if a.multimatch else "unique"
+        if a.joinType == Join.OUTER and not a.multimatch:
+            raise NotImplementedError("outer joins need a multimatch hash table")
+        ht = prog.newHt(kind, J, "jht%d" % J)
+        self.ht = ht
+        idx = ht["index"]
+        prog.hostVars["htcap%d" % idx] = (
+            "L.htcap[%d] = t->ht_capacity[%d] > 0 ? t->ht_capacity[%d] : "
+            "std::max<int64_t>(64, (int64_t)((double)%s * %r * fac(t->ht_factor[%d])));\n"
+            "if (L.htcap[%d] >= (int64_t)0xffffffffll) return DQC_ENOMEM;"
+            % (idx, idx, idx, est, float(a.htSizeFactor), idx, idx))
+        self.storedBuffers = []
+        if kind == "unique":
+            prog.hostVars["mment%d" % idx] = "L.mment[%d] = 0;" % idx
+            tags = prog.buffer("jht%d_tags" % J, CType.UINT64, "L.htcap[%d]" % idx, "0x0ull")
+            ftags = pipe.param("jht%d" % J, "uint64_t*", "B_" + tags)
+            fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
+            count = "L.htcap[%d]" % idx
+        else:
+            prog.hostVars["mment%d" % idx] = (
+                "L.mment[%d] = t->mm_entries[%d] > 0 ? t->mm_entries[%d] : std::max<int64_t>(64, %s * 2);\n"
+                "if (L.mment[%d] >= (int64_t)0xffffffffll) return DQC_ENOMEM;" % (idx, idx, idx, est, idx))
+            dirb = prog.buffer("jht%d_dir" % J, CType.UINT, "2 * L.htcap[%d]" % idx, "0x0ull")
+            sums = prog.buffer("jht%d_sums" % J, CType.UINT64, "(L.htcap[%d] + 4095) / 4096 + 1" % idx)
+            ht["dirBuffer"], ht["sumsBuffer"] = dirb, sums
+            fdir = pipe.param("jht%d_dir" % J, "uint32_t*", "B_" + dirb)
+            fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
+            fecap = pipe.param("jht%d_ecap" % J, "uint32_t", "L.mment[%d]" % idx)
+            count = "L.mment[%d]" % idx
+        fields = []
+        for k, att in enumerate(self.stored):
+            ct = pipe.attrs[att.id].ctype
+            b = prog.buffer("jht%d_p%d" % (J, k), ct, count)
+            self.storedBuffers.append((b, ct))
+            fields.append(pipe.param("jht%d_p%d" % (J, k), "%s*" % ct, "B_" + b))
+        pipe.code.open("if (active)")
+        self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
+        if kind == "unique":
+            pipe.emit("const int64_t js = dqc::ht_insert(%s, %s, jtag%d, jh%d);" % (ftags, fcap, J, J))
+            pipe.code.open("if (js >= 0)")
+            for f, att in zip(fields, self.stored):
+                pipe.emit("%s[js] = %s;" % (f, pipe.access(att)))
+            pipe.code.close()
+            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
+                      % (1 << idx))
+        else:
+            pipe.phased = True
+            pipe.phasedHt = ht
+            pipe.emit("const uint32_t jb = dqc::slot_of(jh%d, %s);" % (J, fcap))
+            pipe.code.open("if constexpr (PHASE == 0)")
+            pipe.emit("atomicAdd(&%s[2u * jb + 1u], 1u);" % fdir)
+            pipe.code.close()
+            pipe.code.open("else")
+            pipe.emit("const uint32_t jpos = %s[2u * jb] + atomicAdd(&%s[2u * jb + 1u], 1u);" % (fdir, fdir))
+            pipe.code.open("if (jpos < %s)" % fecap)
+            for f, att in zip(fields, self.stored):
+                pipe.emit("%s[jpos] = %s;" % (f, pipe.access(att)))
+            pipe.code.close()
+            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); atomicOr(&st->mm_overflow_mask, %du); }"
+                      % (1 << idx))
+            pipe.code.close()
+        pipe.code.close()
+
+    # ----------------------------------------------------------- probe
+    def _storedFields(self, pipe):
+        J = self.algExpr.opId
+        out = []
+        for k, (b, ct) in enumerate(self.storedBuffers):
+            out.append(pipe.param("jht%d_p%d" % (J, k), "const %s*" % ct, "B_" + b))
+        return out
+
+    def _bindBuild(self, pipe, fields, indexReg):
+        """Build-side attributes become lazy loads from the stored columns."""
+        for f, att, (b, ct) in zip(fields, self.stored, self.storedBuffers):
+            pipe.setLazy(att, "%s[%s]" % (f, indexReg), [indexReg], ct)
+
+    def _matchCond(self, ctxt, pipe, fields, indexReg):
+        """C expression: stored keys at indexReg equal the probe keys and the
+        extra join conditions hold.  Loads what it reads explicitly."""
+        a = self.algExpr
+        parts = []
+        if a.multimatch or not self.exact:
+            byId = {att.id: f for f, att in zip(fields, self.stored)}
+            for b, p in zip(self.buildKeys, self.probeKeys):
+                f = byId[b.id]
+                if b.dataType == Type.STRING:
+                    parts.append("dqc::string_equals(%s[%s], %s)" % (f, indexReg, pipe.access(p)))
+                else:
+                    parts.append("(%s[%s] == %s)" % (f, indexReg, pipe.access(p)))
+        if a.conditions is not None:
+            buildIds = set(att.id for att in self.stored)
+            condAttrs = []
+            from dogqc_amd.hipTranslator import _collectAttrs
+            _collectAttrs(a.conditions, condAttrs)
+            for att in condAttrs:
+                st = pipe.attrs.get(att.id)
+                if att.id in buildIds:
+                    pipe.emit("%s = %s;" % (st.reg, st.loader))
+                    st.loaded = True
+            parts.append("(%s)" % ctxt.exprEmitter().emit(a.conditions))
+        return " && ".join(parts) if parts else "true"
+
+    def consumeProbe(self, ctxt):
+        a = self.algExpr
+        pipe = ctxt.pipe
+        prog = ctxt.program
+        J = a.opId
+        idx = self.ht["index"]
+        pipe.emit("// hash join probe (opId %d)" % J)
+        pipe.need(self.probeKeys)
+        condProbe = list(getattr(a, "conditionProbeAttributes", {}).values())
+        pipe.need(condProbe)
+        if self.ht["kind"] == "unique":
+            ftags = pipe.param("jht%d" % J, "const uint64_t*", "B_jht%d_tags" % J)
+            fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
+            fields = self._storedFields(pipe)
+            slot = pipe.reg("jslot%d" % J, "int64_t", "-1")
+            pipe.indexRegs[slot] = "int64_t"
+            self._bindBuild(pipe, fields, slot)
+            pipe.emit("%s = -1;" % slot)
+            pipe.code.open("if (active)")
+            self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+            pipe.emit("uint32_t jc = dqc::probe_start(jh%d, %s), jv = 0;" % (J, fcap))
+            needCheck = (not self.exact) or a.conditions is not None
+            if needCheck:
+                pipe.code.open("for (;;)")
+                pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
+                pipe.emit("if (%s < 0) break;" % slot)
+                pipe.emit("if (%s) break;" % self._matchCond(ctxt, pipe, fields, slot))
+                pipe.code.close()
+            else:
+                pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
+            if a.joinType == Join.INNER:
+                pipe.emit("active = %s >= 0;" % slot)
+            elif a.joinType == Join.SEMI:
+                pipe.emit("active = %s >= 0;" % slot)
+            elif a.joinType == Join.ANTI:
+                pipe.emit("active = %s < 0;" % slot)
+            pipe.code.close()
+            if a.joinType == Join.INNER:
+                for b in self.keyReconstruct:
+                    p = self.probeKeys[self.buildKeys.index(b)]
+                    st = pipe.setComputed(b)
+                    pipe.emit("if (active) %s = %s;" % (st.reg, pipe.access(p)))
+            self.parent.consume(ctxt)
+            return
+
+        fdir = pipe.param("jht%d_dir" % J, "const uint32_t*", "B_jht%d_dir" % J)
+        fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
+        # entries past the capacity were never written (the insert pass
+        # flagged an overflow and the run is repeated): clamp every range
+        self.fecap = pipe.param("jht%d_ecap" % J, "uint32_t", "L.mment[%d]" % idx)
+        fields = self._storedFields(pipe)
+        moff = pipe.reg("moff%d" % J, "uint32_t", "0")
+        pos = pipe.reg("mpos%d" % J, "uint32_t", "0")
+        pipe.indexRegs[pos] = "uint32_t"
+        if a.joinType in (Join.SEMI, Join.ANTI):
+            pipe.code.open("if (active)")
+            self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+            pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
+            pipe.emit("int fm = 0;")
+            pipe.emit("const uint32_t jend = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);" % self.fecap)
+            pipe.code.open("for (%s = jd.x; %s < jend; ++%s)" % (pos, pos, pos))
+            self._bindBuild(pipe, fields, pos)
+            pipe.emit("if (%s) { fm = 1; break; }" % self._matchCond(ctxt, pipe, fields, pos))
+            pipe.code.close()
+            pipe.emit("active = %s;" % ("fm" if a.joinType == Join.SEMI else "!fm"))
+            pipe.code.close()
+            self.parent.consume(ctxt)
+            return
+        if a.joinType == Join.OUTER:
+            raise NotImplementedError("outer joins are not supported by the HIP backend yet")
+
+        probeSide = self.rightChild.algExpr.outRelation
+        probeLive = [att for id, att in a.outRelation.items() if id in probeSide]
+        if EquiJoinTranslator.usePushDownJoin and ctxt.innerLoopCount == 0:
+            self._probePushDown(ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive)
+        else:
+            self._probeLoop(ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive)
+
+    def _probeLoop(self, ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive):
+        """Each lane walks its own match list; the wavefront loops while any
+        lane has matches left (hashJoins.py:465-534)."""
+        J = self.algExpr.opId
+        mend = pipe.reg("mend%d" % J, "uint32_t", "0")
+        pipe.emit("%s = 0; %s = 0;" % (moff, mend))
+        pipe.code.open("if (active)")
+        self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+        pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
+        pipe.emit("%s = jd.x; %s = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);" % (moff, mend, self.fecap))
+        pipe.code.close()
+        pipe.need(probeLive)
+        pipe.emit("int mact%d = active && %s < %s;" % (J, moff, mend))
+        pipe.code.open("while (dqc::ballot(mact%d))" % J)
+        pipe.emit("%s = %s;" % (pos, moff))
+        self._bindBuild(pipe, fields, pos)
+        pipe.emit("active = mact%d;" % J)
+        pipe.emit("if (active) active = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, pos))
+        ctxt.innerLoopCount += 1
+        pipe.innerDepth += 1
+        self.parent.consume(ctxt)
+        pipe.innerDepth -= 1
+        ctxt.innerLoopCount -= 1
+        pipe.emit("++%s;" % moff)
+        pipe.emit("mact%d = mact%d && %s < %s;" % (J, J, moff, mend))
+        pipe.code.close()
+        pipe.emit("active = 0;")
+
+    def _probePushDown(self, ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive):
+        """Push-down parallelism for wave64 (see module docstring)."""
+        J = self.algExpr.opId
+        nw = pipe.bs // 64
+        pipe.staticLds.append("__shared__ uint64_t pdincl%d[%d][64];" % (J, nw))
+        pipe.annotations.append("PD%d" % J)
+        mlen = pipe.reg("mlen%d" % J, "uint64_t", "0")
+        pipe.emit("%s = 0; %s = 0;" % (moff, mlen))
+        pipe.code.open("if (active)")
+        self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+        pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
+        pipe.emit("%s = jd.x; %s = jd.x < %s ? std::min<uint64_t>(jd.y, %s - jd.x) : 0;" % (moff, mlen, self.fecap, self.fecap))
+        pipe.code.close()
+        pipe.need(probeLive)
+        # everything the matching and the operators above read from the probe
+        # tuple moves with it to the lanes that process its matches
+        condProbe = list(getattr(self.algExpr, "conditionProbeAttributes", {}).values())
+        live = pipe.liveState(probeLive + self.probeKeys + condProbe)
+        pipe.emit("const uint64_t pincl%d = dqc::wave_incl_scan(%s);" % (J, mlen))
+        pipe.emit("const uint64_t ptotal%d = dqc::uniform64(dqc::shfl(pincl%d, 63));" % (J, J))
+        pipe.emit("pdincl%d[wave][lane] = pincl%d;" % (J, J))
+        saved = []
+        for r, ct in live.items():
+            s = pipe.reg("%s_pd%d" % (r, J), ct)
+            pipe.emit("%s = %s;" % (s, r))
+            saved.append((r, s))
+        pipe.emit("const uint32_t moffs%d = %s;" % (J, moff))
+        pipe.code.open("for (uint64_t pdone = 0; pdone < ptotal%d; pdone += 64)" % J)
+        pipe.emit("const uint64_t pt = pdone + lane;")
+        pipe.emit("active = pt < ptotal%d;" % J)
+        pipe.emit("int po = 0;")
+        pipe.code.open("if (active)")
+        pipe.emit("int lo = 0, hi = 63;")
+        pipe.emit("while (lo < hi) { const int mid = (lo + hi) >> 1; "
+                  "if (pdincl%d[wave][mid] > pt) hi = mid; else lo = mid + 1; }" % J)
+        pipe.emit("po = lo;")
+        pipe.code.close()
+        pipe.emit("const uint64_t pbase = po ? pdincl%d[wave][po - 1] : 0;" % J)
+        pipe.emit("%s = dqc::shfl(moffs%d, po) + (uint32_t)(pt - pbase);" % (pos, J))
+        for r, s in saved:
+            pipe.emit("%s = dqc::shfl(%s, po);" % (r, s))
+        self._bindBuild(pipe, fields, pos)
+        pipe.emit("if (active) active = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, pos))
+        ctxt.innerLoopCount += 1
+        pipe.innerDepth += 1
+        self.parent.consume(ctxt)
+        pipe.innerDepth -= 1
+        ctxt.innerLoopCount -= 1
+        pipe.code.close()
+        pipe.emit("active = 0;")

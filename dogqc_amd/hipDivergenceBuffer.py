@@ -1,0 +1,101 @@
+"""Lane refill for divergent pipelines, designed for 64-lane wavefronts.
+
+Semantics of the reference's LaneRefill / BufferedLoop
+(dogqc/cudaDivergenceBuffer.py:11-283): at a plan position, tuples that are
+still active are parked in an on-chip buffer, and the operators above that
+position run only once at least ``threshold`` x lanes of the (warp) wavefront
+can be filled; at the end of the input the buffer drains.  Positions inside
+inner loops are skipped (cudaDivergenceBuffer.py:21-24).
+
+gfx950 design: every wavefront owns a 128-entry LDS queue per live register
+(attributes already in registers, plus the row indices of attributes that are
+still loaded lazily, so columns that only later operators read are fetched
+after compaction, for dense lanes only).  Survivors are appended with one
+64-bit ballot and an mbcnt prefix; the downstream code runs on up to 64
+popped entries at a time.  The release threshold is a run-time kernel
+parameter (1..64 lanes), so the autotuner can change it without recompiling.
+"""
+from dogqc_amd.translatorBase import UnaryTranslator
+
+
+class LaneRefill(UnaryTranslator):
+
+    threshold = 0.9
+
+    def __init__(self, parentAlgExpr, child):
+        UnaryTranslator.__init__(self, parentAlgExpr, child)
+        self.threshold = LaneRefill.threshold
+
+    def produce(self, ctxt):
+        self.child.produce(ctxt)
+
+    def opId(self):
+        return self.algExpr.opId + 100000
+
+    def consume(self, ctxt):
+        pipe = ctxt.pipe
+        X = self.algExpr.opId
+        if ctxt.innerLoopCount > 0 or pipe.innerDepth > 0:
+            pipe.emit("// lane refill at opId %d skipped inside an inner loop" % X)
+            self.parent.consume(ctxt)
+            return
+        pipe.annotations.append("LR%d" % X)
+        live = pipe.liveState(self.algExpr.outRelation.values())
+        if pipe.tid not in live:
+            live[pipe.tid] = "int64_t"
+        nw = pipe.bs // 64
+        queues = []
+        for r, ct in live.items():
+            q = "rq%d_%s" % (X, r)
+            pipe.staticLds.append("__shared__ %s %s[%d][128];" % (ct, q, nw))
+            queues.append((r, q))
+        cnt = pipe.reg("rcnt%d" % X, "uint32_t", "0")
+        default = max(1, min(64, int(64 * self.threshold)))
+        pid = pipe.pipeId
+        thr = pipe.param("rthr%d" % X, "uint32_t",
+                         "std::max<int32_t>(1, std::min<int32_t>(64, t->refill_threshold[%d] > 0 ? "
+                         "t->refill_threshold[%d] : %d))" % (pid, pid, default))
+        outer = pipe.refillPoints[-1] if pipe.refillPoints else None
+        pipe.emit("// lane refill (opId %d)" % X)
+        pipe.code.open("")
+        pipe.emit("const uint64_t rm = dqc::ballot(active);")
+        pipe.emit("const uint32_t rpos = %s + dqc::lanes_below(rm);" % cnt)
+        pipe.code.open("if (active)")
+        for r, q in queues:
+            pipe.emit("%s[wave][rpos] = %s;" % (q, r))
+        pipe.code.close()
+        pipe.emit("%s += dqc::popc64(rm);" % cnt)
+        pipe.code.close()
+        if outer is None:
+            pipe.emit("const bool rflush%d = !more;" % X)
+        else:
+            pipe.emit("const bool rflush%d = rflush%d && %s == 0u;" % (X, outer["id"], outer["cnt"]))
+        point = {"id": X, "cnt": cnt}
+        point["mark"] = pipe.code.mark()
+        pipe.code.depth += 1
+        pipe.refillPoints.append(point)
+        pipe.emit("const uint32_t rtake = %s < 64u ? %s : 64u;" % (cnt, cnt))
+        pipe.emit("%s -= rtake;" % cnt)
+        pipe.emit("active = lane < rtake;")
+        pipe.code.open("if (active)")
+        for r, q in queues:
+            pipe.emit("%s = %s[wave][%s + lane];" % (r, q, cnt))
+        pipe.code.close()
+        point["cond"] = "%s >= (rflush%d ? 1u : %s)" % (cnt, X, thr)
+        self.parent.consume(ctxt)
+        pipe.code.close()
+        pipe.emit("active = 0;")
+
+
+def finishRefills(pipe):
+    """Fill in the loop heads of all refill points of a pipeline: while
+    flushing, a loop also runs while any later (downstream) queue still holds
+    tuples."""
+    points = pipe.refillPoints
+    for i, pt in enumerate(points):
+        later = points[i + 1:]
+        cond = pt["cond"]
+        if later:
+            pending = " || ".join("%s > 0u" % q["cnt"] for q in later)
+            cond = "(%s) || (rflush%d && (%s))" % (cond, pt["id"], pending)
+        pipe.code.fill(pt["mark"], "while (%s) {" % cond)

@@ -1,0 +1,112 @@
+"""Scalar expression -> HIP C++ expression text.
+
+Literal spelling follows the reference so that arithmetic has identical C
+semantics (scalarAlgebra.py:142-154): INT/DATE/DOUBLE tokens verbatim, FLOAT
+tokens as ``(float)token``, CHAR as a character literal, STRING as a pointer
+range into a device-side constant array.
+"""
+import dogqc_amd.scalarAlgebra as scal
+from dogqc_amd.types import Type
+from dogqc_amd.hiplang import langType, cLiteralString
+
+
+class ExprEmitter(object):
+
+    def __init__(self, program, decimalRepr, access):
+        self.program = program
+        self.decimalRepr = decimalRepr
+        self.access = access  # attr -> C expression of its register
+
+    def emit(self, e):
+        m = getattr(self, "_" + e.__class__.__name__, None)
+        if m is None:
+            raise NotImplementedError("scalar expression " + e.__class__.__name__)
+        return m(e)
+
+    def _AttrExpr(self, e):
+        return self.access(e.attr)
+
+    def _ConstExpr(self, e):
+        t = e.type
+        if t in (Type.INT, Type.DATE, Type.DOUBLE, Type.INT64):
+            return "(" + e.token + ")"
+        if t == Type.FLOAT:
+            return "((float)" + e.token + ")"
+        if t == Type.CHAR:
+            if len(e.token) != 1:
+                raise SyntaxError("CHAR constant must be one character: %r" % e.token)
+            return "((int8_t)%d)" % ord(e.token)
+        if t == Type.STRING:
+            return self.program.stringConstant(e.token)
+        raise NotImplementedError("constant of type " + str(t))
+
+    def _bin(self, e, op):
+        return "(%s %s %s)" % (self.emit(e.leftChild), op, self.emit(e.rightChild))
+
+    def _EqualsExpr(self, e):
+        if e.leftChild.type == Type.STRING:
+            return "dqc::string_equals(%s, %s)" % (self.emit(e.leftChild), self.emit(e.rightChild))
+        return self._bin(e, "==")
+
+    def _LikeExpr(self, e):
+        return "dqc::string_like(%s, %s)" % (self.emit(e.leftChild), self.emit(e.rightChild))
+
+    def _SmallerExpr(self, e):
+        return self._bin(e, "<")
+
+    def _SmallerEqualExpr(self, e):
+        return self._bin(e, "<=")
+
+    def _LargerExpr(self, e):
+        return self._bin(e, ">")
+
+    def _LargerEqualExpr(self, e):
+        return self._bin(e, ">=")
+
+    def _AndExpr(self, e):
+        return self._bin(e, "&&")
+
+    def _OrExpr(self, e):
+        return self._bin(e, "||")
+
+    def _NotExpr(self, e):
+        return "(!%s)" % self.emit(e.child)
+
+    def _MulExpr(self, e):
+        return self._bin(e, "*")
+
+    def _DivExpr(self, e):
+        return self._bin(e, "/")
+
+    def _AddExpr(self, e):
+        return self._bin(e, "+")
+
+    def _SubExpr(self, e):
+        return self._bin(e, "-")
+
+    def _AbsExpr(self, e):
+        fn = "fabs" if e.child.type in (Type.FLOAT, Type.DOUBLE) else "abs"
+        return "%s(%s)" % (fn, self.emit(e.child))
+
+    def _ExtractExpr(self, e):
+        c = self.emit(e.child)
+        if e.extract == scal.ExtractType.DAY:
+            return "(%s %% 100)" % c
+        if e.extract == scal.ExtractType.MONTH:
+            return "((%s / 100) %% 100)" % c
+        return "(%s / 10000)" % c
+
+    def _SubstringExpr(self, e):
+        return "dqc::string_substring(%s, %s, %s)" % (self.emit(e.exprString), self.emit(e.exprFrom),
+                                                      self.emit(e.exprFor))
+
+    def _CaseExpr(self, e):
+        ct = langType(Type.FLOAT, self.decimalRepr)
+        out = "((%s)%s)" % (ct, self.emit(e.exprElse)) if e.exprElse is not None else "((%s)0)" % ct
+        for w, t in reversed(e.exprListWhenThen):
+            out = "(%s ? (%s)%s : %s)" % (self.emit(w), ct, self.emit(t), out)
+        return out
+
+
+def stringConstantDecl(name, token):
+    return "static __device__ const char %s[] = %s;" % (name, cLiteralString(token))

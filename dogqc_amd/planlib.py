@@ -1,0 +1,200 @@
+"""Named plans: the workloads and the feature plans the GPU tests run.
+
+Each entry builds a plan with the reference plan API and carries the
+translator knobs it is compiled with.  ``translate(name, acc)`` resolves and
+translates a plan; ``build()`` in the repository's entry module compiles every
+entry ahead of time so no compilation happens on the GPU machine.
+"""
+from collections import OrderedDict
+import contextlib
+
+import numpy as np
+
+import dogqc_amd.scalarAlgebra as scal
+from dogqc_amd import workloads
+from dogqc_amd.relationalAlgebra import RelationalAlgebra, Reduction
+from dogqc_amd.types import Type
+from dogqc_amd.dbio import MemoryAccessor
+from dogqc_amd.hiplang import CType
+from dogqc_amd.hipTranslator import HipCompiler
+from dogqc_amd.hashJoins import EquiJoinTranslator
+from dogqc_amd.hipDivergenceBuffer import LaneRefill
+from dogqc_amd.kernel import KernelCall
+
+
+@contextlib.contextmanager
+def knobs(pushdown=False, block=None, threshold=None):
+    old = (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold)
+    EquiJoinTranslator.usePushDownJoin = pushdown
+    if block is not None:
+        KernelCall.defaultBlockSize = block
+    if threshold is not None:
+        LaneRefill.threshold = threshold
+    try:
+        yield
+    finally:
+        EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold = old
+
+
+# ------------------------------------------------------------- feature plans
+# A small multi-type table "t" and a dimension "u" for joins, LIKE, MIN/MAX,
+# semi/anti joins and hashed (non-packable) group keys.
+
+FEATURE_SCHEMA = OrderedDict([
+    ("t", OrderedDict([("t_id", Type.INT), ("t_grp", Type.INT), ("t_cat", Type.CHAR), ("t_val", Type.DOUBLE),
+                       ("t_f", Type.FLOAT), ("t_big", Type.INT64), ("t_date", Type.DATE), ("t_txt", Type.STRING)])),
+    ("u", OrderedDict([("u_id", Type.INT), ("u_grp", Type.INT), ("u_w", Type.DOUBLE), ("u_name", Type.STRING)])),
+])
+
+WORDS = ["alpha", "bravo", "charlie", "delta", "echo", "foxtrot", "golf", "hotel", "india", "juliet"]
+
+
+def featureData(nt=5000, nu=700, seed=7):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = OrderedDict()
+    t["t_id"] = (Type.INT, np.arange(nt, dtype=np.int32))
+    t["t_grp"] = (Type.INT, rng.integers(-50, 50, size=nt).astype(np.int32))
+    t["t_cat"] = (Type.CHAR, rng.integers(ord("a"), ord("e"), size=nt).astype(np.int8))
+    t["t_val"] = (Type.DOUBLE, np.round(rng.uniform(-100, 100, size=nt), 2))
+    t["t_f"] = (Type.FLOAT, np.round(rng.uniform(0, 10, size=nt), 1).astype(np.float32))
+    t["t_big"] = (Type.INT64, rng.integers(-(1 << 40), 1 << 40, size=nt, dtype=np.int64))
+    t["t_date"] = (Type.DATE, workloads.dateTable()[rng.integers(0, 2557, size=nt)])
+    txt = [(" ".join(WORDS[j] for j in rng.integers(0, len(WORDS), size=rng.integers(0, 4)))).encode()
+           for _ in range(nt)]
+    t["t_txt"] = (Type.STRING, txt)
+    u = OrderedDict()
+    u["u_id"] = (Type.INT, rng.permutation(nu).astype(np.int32) * 3)
+    u["u_grp"] = (Type.INT, rng.integers(-50, 50, size=nu).astype(np.int32))
+    u["u_w"] = (Type.DOUBLE, np.round(rng.uniform(0, 5, size=nu), 3))
+    u["u_name"] = (Type.STRING, [WORDS[j].encode() for j in rng.integers(0, len(WORDS), size=nu)])
+    return _asAccessor(OrderedDict([("t", t), ("u", u)]))
+
+
+def _asAccessor(tables):
+    conv = OrderedDict()
+    for name, cols in tables.items():
+        conv[name] = OrderedDict()
+        for a, (typ, data) in cols.items():
+            if typ == Type.STRING and not isinstance(data, tuple):
+                lens = np.fromiter((len(s) for s in data), dtype=np.uint64, count=len(data))
+                offs = np.zeros(len(data) + 1, dtype=np.uint64)
+                np.cumsum(lens, out=offs[1:])
+                chars = np.frombuffer(b"".join(data) or b"\0", dtype=np.uint8)
+                data = (offs, chars)
+            conv[name][a] = (typ, data)
+    return MemoryAccessor(conv)
+
+
+def featureEmpty():
+    return featureData(0, 0)
+
+
+def _A(n):
+    return scal.AttrExpr(n)
+
+
+def _C(tok, typ):
+    return scal.ConstExpr(tok, typ)
+
+
+def _featurePlan(name, alg):
+    if name == "f_scan_minmax":
+        # no-group MIN/MAX/SUM/COUNT/AVG over a filtered scan with INT64 and FLOAT inputs
+        p = alg.aggregation([], [(Reduction.MIN, "t_val", "mn"), (Reduction.MAX, "t_val", "mx"),
+                                 (Reduction.SUM, "t_big", "sb"), (Reduction.MAX, "t_big", "mb"),
+                                 (Reduction.AVG, "t_f", "af"), (Reduction.COUNT, "", "c")],
+                            alg.selection(scal.LargerExpr(_A("t_date"), _C("19950101", Type.DATE)), alg.scan("t")))
+        return p, {}
+    if name == "f_groupby_hashed":
+        # INT64 + CHAR group key does not pack into 63 bits -> hashed global table
+        p = alg.aggregation(["t_big2", "t_cat"], [(Reduction.SUM, "t_val", "s"), (Reduction.COUNT, "", "c")],
+                            alg.map("t_big2", scal.DivExpr(_A("t_big"), _C("1099511627776", Type.INT64)),
+                                    alg.scan("t")))
+        return p, {3: {"numgroups": 8}}
+    if name == "f_groupby_wide":
+        # many groups: (t_grp, t_cat) with an under-estimated numgroups -> overflow + retry
+        p = alg.aggregation(["t_grp", "t_cat"], [(Reduction.SUM, "t_val", "s"), (Reduction.MIN, "t_id", "mi"),
+                                                  (Reduction.MAX, "t_date", "md"), (Reduction.COUNT, "", "c")],
+                            alg.scan("t"))
+        return p, {2: {"numgroups": 4}}
+    if name == "f_like_project":
+        # LIKE filter + string materialisation + map arithmetic
+        p = alg.projection(["t_id", "t_txt", "v2"],
+                           alg.map("v2", scal.MulExpr(_A("t_val"), _C("2", Type.INT)),
+                                   alg.selection(scal.LikeExpr(_A("t_txt"), _C("%echo%", Type.STRING)),
+                                                 alg.scan("t"))))
+        return p, {2: {"selectivity": 0.01}}
+    if name == "f_join_unique":
+        # PK join on u_id (unique), build payload + string, grouped by a build attribute
+        p = alg.aggregation(["u_name"], [(Reduction.SUM, "w", "sw"), (Reduction.COUNT, "", "c")],
+                            alg.map("w", scal.MulExpr(_A("u_w"), _A("t_val")),
+                                    alg.join(("u_id", "t_id"), alg.scan("u"),
+                                             alg.selection(scal.EqualsExpr(_A("t_cat"), _C("b", Type.CHAR)),
+                                                           alg.scan("t")))))
+        return p, {4: {"multimatch": False}, 6: {"numgroups": 16}}
+    if name == "f_join_multi":
+        # multimatch join on grp (many duplicates both sides), result rows materialised
+        p = alg.projection(["t_id", "u_id", "t_grp"],
+                           alg.join(("u_grp", "t_grp"), alg.scan("u"),
+                                    alg.selection(scal.SmallerExpr(_A("t_id"), _C("300", Type.INT)), alg.scan("t"))))
+        return p, {4: {"multimatch": True}}
+    if name == "f_semi":
+        p = alg.aggregation(["t_cat"], [(Reduction.COUNT, "", "c")],
+                            alg.semijoin(("u_grp", "t_grp"), None, alg.scan("u"), alg.scan("t")))
+        return p, {3: {"multimatch": False}}
+    if name == "f_anti_multi":
+        p = alg.aggregation(["t_cat"], [(Reduction.COUNT, "", "c"), (Reduction.SUM, "t_val", "s")],
+                            alg.antijoin(("u_grp", "t_grp"),
+                                         scal.LargerExpr(_A("u_w"), _C("2.5", Type.DOUBLE)),
+                                         alg.scan("u"), alg.scan("t")))
+        return p, {3: {"multimatch": True}}
+    raise KeyError(name)
+
+
+FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
+                 "f_join_multi", "f_semi", "f_anti_multi"]
+
+# name -> (workload or "feature", options)
+PLANS = OrderedDict()
+PLANS["c0"] = ("c0", {})
+PLANS["c1"] = ("c1", {})
+PLANS["c2_none"] = ("c2", {})
+PLANS["c2_refill"] = ("c2", {"buffers": [3]})
+PLANS["c2_pushdown"] = ("c2", {"pushdown": True})
+PLANS["c2_both"] = ("c2", {"buffers": [3], "pushdown": True})
+for _n in FEATURE_PLANS:
+    PLANS[_n] = ("feature", {})
+PLANS["f_join_multi_pushdown"] = ("feature", {"pushdown": True, "base": "f_join_multi"})
+PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_project"})
+
+
+def build(name, acc):
+    """(alg, plan, cfg, options) for a named plan over an accessor."""
+    kind, opts = PLANS[name]
+    if kind == "feature":
+        alg = RelationalAlgebra(acc)
+        p, cfg = _featurePlan(opts.get("base", name), alg)
+        return alg, p, cfg, opts
+    alg, p, cfg = workloads.plan(kind, acc)
+    return alg, p, cfg, opts
+
+
+def emptyAccessor(name):
+    kind, _ = PLANS[name]
+    return featureEmpty() if kind == "feature" else workloads.emptyAccessor(kind)
+
+
+def translate(name, acc=None, decimalRepr=CType.FP64):
+    """Resolve and translate a named plan; returns the HipCompiler with
+    generated source (call .compile() to get the module)."""
+    acc = acc or emptyAccessor(name)
+    alg, p, cfg, opts = build(name, acc)
+    plan = alg.resolveAlgebraPlan(p, cfg)
+    with knobs(pushdown=opts.get("pushdown", False), block=opts.get("block"), threshold=opts.get("threshold")):
+        comp = HipCompiler(alg, "gfx950", decimalRepr)
+        comp.setBuffers(opts.get("buffers", []))
+        tp = alg.translateToCompilerPlan(plan, comp)
+        comp.gencode(tp)
+    comp.planName = name
+    comp.resolvedPlan = plan
+    return comp

@@ -1,0 +1,92 @@
+"""Execution of a compiled plan for ``HipCompiler.execute()``.
+
+Binds the plan's input columns from the algebra context's database accessor,
+runs the module through the runtime (device-resident inputs, retry on
+overflow), and prints the reference's text protocol: ``Result: N tuples``,
+up to ten sample rows and a ``<timing>`` block with one line per kernel
+launch (codegen.py:153-156, cudaTranslator.py:607-643), plus ``<pN>``
+lane-activity blocks when profilers are placed (cudaLaneActivityProfile.py).
+"""
+from collections import OrderedDict
+
+import numpy as np
+
+from dogqc_amd import runtime as rt
+
+
+def decodeStrings(col, chars):
+    out = np.empty(len(col), dtype=object)
+    b = np.asarray(chars, dtype=np.uint8).tobytes()
+    for i, (s, e) in enumerate(zip(col["start"], col["end"])):
+        out[i] = b[int(s):int(e)]
+    return out
+
+
+def resultColumns(module, execution, accessor):
+    """Result as OrderedDict name -> numpy array, strings decoded to bytes."""
+    cols = execution.columns()
+    desc = module.desc
+    byIndex = {c["index"]: c for c in desc["columns"]}
+    out = OrderedDict()
+    for c, (name, arr) in zip(desc["result"], cols.items()):
+        if c["ctype"] == "dqc::str_offs":
+            src = byIndex[c["char_col"]]
+            chars = accessor.column(src["table"], src["attr"])[1]
+            out[name] = decodeStrings(arr, chars)
+        else:
+            out[name] = arr
+    return out
+
+
+def timeSteps(module, inputs, tuning, stream, ws):
+    """Per-launch milliseconds of one enqueue (events between launches)."""
+    steps = module.steps()
+    events = [rt.Event() for _ in range(len(steps) + 1)]
+    rt.enqueue(module, inputs, tuning, ws, stream, events)
+    stream.sync()
+    return [(name, events[i].elapsed(events[i + 1])) for i, name in enumerate(steps)]
+
+
+def executeAndPrint(compiler, printRows=10):
+    acc = compiler.algebraContext.ctxt.database
+    module = rt.PlanModule(compiler.module)
+    inputs = rt.BoundInputs(module, acc)
+    tuning = rt.newTuning()
+    if module.desc["profiles"]:
+        tuning.profile = 1
+    stream = rt.Stream()
+    ex = rt.run(module, inputs, tuning, stream)
+    res = resultColumns(module, ex, acc)
+    lines = ["", "Result: %d tuples" % ex.count]
+    n = min(printRows, ex.count)
+    for i in range(n):
+        parts = []
+        for name, arr in res.items():
+            v = arr[i]
+            if isinstance(v, bytes):
+                s = v.decode("latin-1")
+            elif isinstance(v, (float, np.floating)):
+                s = "%15.2f" % v
+            else:
+                s = "%10d" % int(v)
+            parts.append("%s: %s" % (name, s))
+        lines.append("  ".join(parts))
+    if ex.count > printRows:
+        lines.append("[...]")
+    for p, hist in ex.profile().items():
+        lines.append("<p%d>" % p)
+        lines.extend("%d, %d" % (i, int(c)) for i, c in enumerate(hist))
+        lines.append("</p%d>" % p)
+    ws = rt.DeviceBuffer(ex.res.ws_bytes)
+    times = timeSteps(module, inputs, tuning, stream, ws)
+    lines.append("<timing>")
+    total = 0.0
+    for name, ms in times:
+        lines.append("%32s: %6.3f ms" % (name, ms))
+        total += ms
+    lines.append("%32s: %6.3f ms" % ("totalKernelTime", total))
+    lines.append("</timing>")
+    out = "\n".join(lines) + "\n"
+    print(out)
+    ex.release()
+    return out, res

@@ -1,0 +1,326 @@
+"""ctypes binding of libdqcrt.so and of plan modules (include/dogqc_amd.h).
+
+This is the Python side of the C ABI: it binds numpy/mmdb columns to a plan
+module's input slots, keeps them resident in device memory, runs the plan
+with the runtime's retry driver and copies result columns back.  There is no
+fallback: if the shared library or a GPU is missing, calls raise.
+"""
+import ctypes as C
+import json
+import os
+from collections import OrderedDict
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "libdqcrt.so")
+
+MAX_PIPES, MAX_HTS, MAX_RESULT_COLS, PROFILE_BUCKETS = 32, 32, 64, 65
+
+RUNTIME_SYMBOLS = [
+    "dqc_last_error", "dqc_device_init", "dqc_malloc", "dqc_free", "dqc_memcpy_h2d", "dqc_memcpy_d2h",
+    "dqc_memset", "dqc_stream_create", "dqc_stream_destroy", "dqc_stream_sync", "dqc_event_create",
+    "dqc_event_destroy", "dqc_event_elapsed_ms", "dqc_event_record", "dqc_mmdb_load", "dqc_module_load",
+    "dqc_module_unload", "dqc_module_describe", "dqc_query_layout", "dqc_query_enqueue", "dqc_query_run",
+    "dqc_query_release", "dqc_graph_create", "dqc_graph_launch", "dqc_graph_destroy",
+]
+PLAN_SYMBOLS = ["dqc_plan_describe", "dqc_plan_layout_of", "dqc_plan_enqueue"]
+
+
+class Inputs(C.Structure):
+    _fields_ = [("ntables", C.c_int32), ("ncols", C.c_int32), ("table_rows", C.POINTER(C.c_int64)),
+                ("cols", C.POINTER(C.c_void_p))]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("grid", C.c_int32 * MAX_PIPES), ("refill_threshold", C.c_int32 * MAX_PIPES),
+                ("lds_slots", C.c_int32 * MAX_PIPES), ("lds_replicas", C.c_int32 * MAX_PIPES),
+                ("ht_factor", C.c_float * MAX_HTS), ("ht_capacity", C.c_int64 * MAX_HTS),
+                ("mm_entries", C.c_int64 * MAX_HTS), ("result_capacity", C.c_int64),
+                ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+class Status(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("ht_overflow_mask", C.c_uint32), ("mm_overflow_mask", C.c_uint32),
+                ("reserved", C.c_uint32), ("result_count", C.c_uint64), ("mm_total", C.c_uint64 * MAX_HTS)]
+
+
+class Layout(C.Structure):
+    _fields_ = [("ws_bytes", C.c_uint64), ("nsteps", C.c_int32), ("nhts", C.c_int32),
+                ("ht_capacity", C.c_int64 * MAX_HTS), ("mm_entries", C.c_int64 * MAX_HTS),
+                ("result_capacity", C.c_int64), ("nresult", C.c_int32), ("nprofile", C.c_int32),
+                ("result_offset", C.c_uint64 * MAX_RESULT_COLS), ("profile_offset", C.c_uint64)]
+
+
+class QueryResult(C.Structure):
+    _fields_ = [("ws", C.c_void_p), ("ws_bytes", C.c_uint64), ("layout", Layout), ("status", Status),
+                ("retries", C.c_int32)]
+
+
+class DqcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libdqcrt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise DqcError("libdqcrt.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        L.dqc_last_error.restype = C.c_char_p
+        L.dqc_module_describe.restype = C.c_char_p
+        vp = C.c_void_p
+        L.dqc_device_init.argtypes = [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
+        L.dqc_malloc.argtypes = [C.POINTER(vp), C.c_uint64]
+        L.dqc_free.argtypes = [vp]
+        L.dqc_memcpy_h2d.argtypes = [vp, vp, C.c_uint64]
+        L.dqc_memcpy_d2h.argtypes = [vp, vp, C.c_uint64]
+        L.dqc_memset.argtypes = [vp, C.c_int, C.c_uint64]
+        L.dqc_stream_create.argtypes = [C.POINTER(vp)]
+        L.dqc_stream_destroy.argtypes = [vp]
+        L.dqc_stream_sync.argtypes = [vp]
+        L.dqc_event_create.argtypes = [C.POINTER(vp)]
+        L.dqc_event_destroy.argtypes = [vp]
+        L.dqc_event_record.argtypes = [vp, vp]
+        L.dqc_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
+        L.dqc_mmdb_load.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(C.c_uint64)]
+        L.dqc_module_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.dqc_module_unload.argtypes = [vp]
+        L.dqc_module_describe.argtypes = [vp]
+        L.dqc_query_layout.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), C.POINTER(Layout)]
+        L.dqc_query_enqueue.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), vp, C.c_uint64, vp,
+                                        C.POINTER(vp)]
+        L.dqc_query_run.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), vp, C.c_int32, C.POINTER(QueryResult)]
+        L.dqc_query_release.argtypes = [C.POINTER(QueryResult)]
+        L.dqc_graph_create.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), vp, C.c_uint64, vp, C.POINTER(vp)]
+        L.dqc_graph_launch.argtypes = [vp, vp]
+        L.dqc_graph_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().dqc_last_error().decode(errors="replace")
+        raise DqcError("%s failed (%d): %s" % (what, rc, msg))
+
+
+_device = None
+
+
+def device(index=0):
+    """Initialise the device once; returns (compute units, HBM bytes)."""
+    global _device
+    if _device is None:
+        cus, hbm = C.c_int32(), C.c_uint64()
+        check(lib().dqc_device_init(index, C.byref(cus), C.byref(hbm)), "dqc_device_init")
+        _device = (cus.value, hbm.value)
+    return _device
+
+
+class DeviceBuffer(object):
+    """Owned device allocation."""
+
+    def __init__(self, nbytes):
+        device()
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        check(lib().dqc_malloc(C.byref(self.ptr), self.nbytes), "dqc_malloc")
+
+    @staticmethod
+    def fromArray(arr):
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(max(arr.nbytes, 1))
+        if arr.nbytes:
+            check(lib().dqc_memcpy_h2d(b.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes), "dqc_memcpy_h2d")
+        b.dtype, b.count = arr.dtype, len(arr)
+        return b
+
+    def toArray(self, dtype, count, offset=0):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            src = C.c_void_p(self.ptr.value + offset)
+            check(lib().dqc_memcpy_d2h(out.ctypes.data_as(C.c_void_p), src, out.nbytes), "dqc_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().dqc_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def copyToHost(ptr, dtype, count):
+    out = np.empty(count, dtype=dtype)
+    if out.nbytes:
+        check(lib().dqc_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), out.nbytes), "dqc_memcpy_d2h")
+    return out
+
+
+class Stream(object):
+
+    def __init__(self):
+        device()
+        self.handle = C.c_void_p()
+        check(lib().dqc_stream_create(C.byref(self.handle)), "dqc_stream_create")
+
+    def sync(self):
+        check(lib().dqc_stream_sync(self.handle), "dqc_stream_sync")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().dqc_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event(object):
+
+    def __init__(self):
+        self.handle = C.c_void_p()
+        check(lib().dqc_event_create(C.byref(self.handle)), "dqc_event_create")
+
+    def record(self, stream):
+        check(lib().dqc_event_record(self.handle, stream.handle), "dqc_event_record")
+
+    def elapsed(self, other):
+        ms = C.c_float()
+        check(lib().dqc_event_elapsed_ms(self.handle, other.handle, C.byref(ms)), "dqc_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().dqc_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+CTYPE_DTYPE = {"int32_t": np.int32, "uint32_t": np.uint32, "int64_t": np.int64, "uint64_t": np.uint64,
+               "float": np.float32, "double": np.float64, "int8_t": np.int8, "char": np.uint8,
+               "dqc::str_offs": np.dtype([("start", np.uint64), ("end", np.uint64)])}
+
+
+class PlanModule(object):
+    """A loaded plan module."""
+
+    def __init__(self, path):
+        device()
+        self.path = path
+        self.handle = C.c_void_p()
+        check(lib().dqc_module_load(path.encode(), C.byref(self.handle)), "dqc_module_load")
+        self.desc = json.loads(lib().dqc_module_describe(self.handle).decode())
+
+    def steps(self):
+        return self.desc["steps"]
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().dqc_module_unload(self.handle)
+        except Exception:
+            pass
+
+
+class BoundInputs(object):
+    """Device-resident inputs of one plan: columns uploaded from an accessor
+    (``column(table, attr)`` -> array or (offsets, chars)) or given device
+    pointers.  Uploaded buffers are cached per (table, attr) in ``store``."""
+
+    def __init__(self, module, accessor, store=None, rows=None):
+        self.module = module
+        self.store = {} if store is None else store
+        desc = module.desc
+        self.rows = np.zeros(max(1, len(desc["tables"])), dtype=np.int64)
+        for i, t in enumerate(desc["tables"]):
+            self.rows[i] = accessor.schema[t]["size"] if rows is None else rows[t]
+        self.ptrs = (C.c_void_p * max(1, len(desc["columns"])))()
+        for c in desc["columns"]:
+            key = (c["table"], c["attr"], c["part"])
+            if key not in self.store:
+                data = accessor.column(c["table"], c["attr"])
+                if c["part"] == "offset":
+                    data = data[0]
+                elif c["part"] == "char":
+                    data = data[1]
+                self.store[key] = DeviceBuffer.fromArray(np.ascontiguousarray(data))
+            self.ptrs[c["index"]] = self.store[key].ptr
+        self.struct = Inputs(len(desc["tables"]), len(desc["columns"]),
+                             self.rows.ctypes.data_as(C.POINTER(C.c_int64)),
+                             C.cast(self.ptrs, C.POINTER(C.c_void_p)))
+
+
+def newTuning():
+    return Tuning()
+
+
+class Execution(object):
+    """Result of one plan run: owns the workspace until released."""
+
+    def __init__(self, module, inputs, tuning, res):
+        self.module, self.inputs, self.tuning, self.res = module, inputs, tuning, res
+
+    @property
+    def count(self):
+        return int(self.res.status.result_count)
+
+    def columns(self):
+        """Result columns as an OrderedDict name -> numpy array."""
+        out = OrderedDict()
+        n = min(self.count, int(self.res.layout.result_capacity))
+        for i, c in enumerate(self.module.desc["result"]):
+            dt = CTYPE_DTYPE[c["ctype"]]
+            out[c["name"]] = copyToHost(self.res.ws + self.res.layout.result_offset[i], dt, n)
+        return out
+
+    def profile(self):
+        lay = self.res.layout
+        if lay.nprofile == 0:
+            return {}
+        arr = copyToHost(self.res.ws + lay.profile_offset, np.uint64, lay.nprofile * PROFILE_BUCKETS)
+        return {p: arr[i * PROFILE_BUCKETS:(i + 1) * PROFILE_BUCKETS]
+                for i, p in enumerate(self.module.desc["profiles"])}
+
+    def release(self):
+        if self.res.ws:
+            lib().dqc_query_release(C.byref(self.res))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def run(module, inputs, tuning=None, stream=None, max_retries=8):
+    tuning = tuning or Tuning()
+    stream = stream or Stream()
+    res = QueryResult()
+    check(lib().dqc_query_run(module.handle, C.byref(inputs.struct), C.byref(tuning), stream.handle,
+                              max_retries, C.byref(res)), "dqc_query_run")
+    return Execution(module, inputs, tuning, res)
+
+
+def layout(module, inputs, tuning):
+    lay = Layout()
+    check(lib().dqc_query_layout(module.handle, C.byref(inputs.struct), C.byref(tuning), C.byref(lay)),
+          "dqc_query_layout")
+    return lay
+
+
+def enqueue(module, inputs, tuning, ws, stream, events=None):
+    evs = None
+    if events is not None:
+        evs = (C.c_void_p * len(events))(*[e.handle.value for e in events])
+    check(lib().dqc_query_enqueue(module.handle, C.byref(inputs.struct), C.byref(tuning), ws.ptr, ws.nbytes,
+                                  stream.handle, evs), "dqc_query_enqueue")

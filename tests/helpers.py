@@ -1,0 +1,64 @@
+"""Result comparison for parity tests: results are multisets of rows.
+
+Integer, string and non-aggregated floating-point columns must match bit for
+bit; floating-point SUM/AVG aggregates (whose GPU summation order is not
+fixed) within ``rtol`` relative error (1e-9 by default) plus ``atol``.
+"""
+import numpy as np
+
+import dogqc_amd.relationalAlgebra as ra
+from dogqc_amd.relationalAlgebra import Reduction
+
+
+def toleranceColumns(plan):
+    """Names of floating-point SUM/AVG aggregate outputs in a resolved plan."""
+    names = set()
+
+    def walk(node):
+        if isinstance(node, ra.Aggregation):
+            for id, (inId, red) in node.aggregateTuples.items():
+                if red in (Reduction.SUM, Reduction.AVG):
+                    names.add(node.aggregateAttributes[id].name)
+        for c in node.children():
+            walk(c)
+    for root in plan:
+        walk(root)
+    return names
+
+
+def _sortKey(cols, names):
+    keys = []
+    for n in reversed(names):
+        c = cols[n]
+        if c.dtype == object:
+            _, inv = np.unique(c.astype(bytes), return_inverse=True) if len(c) else (None, np.zeros(0, int))
+            keys.append(inv)
+        else:
+            keys.append(c)
+    return np.lexsort(keys) if keys else np.arange(0)
+
+
+def assertSameResult(got, want, tolCols=(), rtol=1e-9, atol=0.0, label=""):
+    assert list(got) == list(want), "%s: columns %s != %s" % (label, list(got), list(want))
+    n = len(next(iter(want.values()))) if want else 0
+    for k in got:
+        assert len(got[k]) == n, "%s: %s has %d rows, expected %d" % (label, k, len(got[k]), n)
+    exact = [k for k in want if k not in tolCols]
+    og, ow = _sortKey(got, exact), _sortKey(want, exact)
+    for k in want:
+        g, w = np.asarray(got[k])[og], np.asarray(want[k])[ow]
+        if k in tolCols:
+            g, w = g.astype(np.float64), w.astype(np.float64)
+            bad = ~(np.isclose(g, w, rtol=rtol, atol=atol) | (np.isnan(g) & np.isnan(w)))
+            assert not bad.any(), "%s: column %s differs beyond rtol=%g at %d rows, e.g. %r vs %r" % (
+                label, k, rtol, int(bad.sum()), g[bad][:3], w[bad][:3])
+        elif w.dtype == object:
+            assert list(g) == list(w), "%s: column %s differs" % (label, k)
+        elif w.dtype.kind == "f":
+            same = (g.view(np.uint64 if g.itemsize == 8 else np.uint32) ==
+                    w.astype(g.dtype).view(np.uint64 if g.itemsize == 8 else np.uint32)) | (np.isnan(g) & np.isnan(w))
+            assert same.all(), "%s: column %s not bit-identical at %d rows, e.g. %r vs %r" % (
+                label, k, int((~same).sum()), g[~same][:3], w[~same][:3])
+        else:
+            assert np.array_equal(g, w.astype(g.dtype)), "%s: column %s differs at %d rows" % (
+                label, k, int((g != w.astype(g.dtype)).sum()))

@@ -1,0 +1,57 @@
+"""The C ABI: libdqcrt.so and plan modules load and export every function
+include/dogqc_amd.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from dogqc_amd import runtime as rt, planlib, build
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dogqc_amd.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dqc_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def runtimeLib():
+    if not os.path.exists(rt.LIBPATH):
+        build.buildRuntime()
+    return ctypes.CDLL(rt.LIBPATH)
+
+
+def test_header_lists_runtime_and_plan_abi():
+    names = declared()
+    assert set(rt.PLAN_SYMBOLS) <= set(names)
+    assert set(rt.RUNTIME_SYMBOLS) <= set(names)
+    assert set(names) == set(rt.PLAN_SYMBOLS) | set(rt.RUNTIME_SYMBOLS)
+
+
+def test_runtime_exports(runtimeLib):
+    for s in rt.RUNTIME_SYMBOLS:
+        assert hasattr(runtimeLib, s), s
+
+
+def test_plan_module_exports_and_describe():
+    comp = planlib.translate("c1")
+    path = comp.compile("c1")
+    lib = ctypes.CDLL(path)
+    for s in rt.PLAN_SYMBOLS:
+        assert hasattr(lib, s), s
+    lib.dqc_plan_describe.restype = ctypes.c_char_p
+    import json
+    desc = json.loads(lib.dqc_plan_describe().decode())
+    assert desc["tables"] == ["dim", "fact"]
+    assert [c["attr"] for c in desc["columns"]] == ["d_key", "d_cat", "f_fk", "f_price", "f_disc", "f_date"]
+    assert [c["name"] for c in desc["result"]] == ["d_key", "revenue"]
+
+
+def test_struct_sizes_match_header():
+    # the ctypes mirrors must have the C layout
+    assert ctypes.sizeof(rt.Tuning) == 4 * 32 * 4 + 4 * 32 + 8 * 32 * 2 + 8 + 4 + 4 * 7
+    assert ctypes.sizeof(rt.Status) == 16 + 8 + 8 * 32

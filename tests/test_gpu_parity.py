@@ -1,0 +1,174 @@
+"""GPU parity: every named plan, run through the C ABI on the HIP path,
+against the CPU oracle (oracle/planeval.py) on the same seeded inputs.
+
+Integer, string and per-row floating-point columns are bit-exact; FP64
+SUM/AVG aggregates within 1e-9 relative (tests/helpers.py).
+"""
+import numpy as np
+import pytest
+
+from dogqc_amd import planlib, workloads, runtime as rt, runner
+from dogqc_amd.dbio import MemoryAccessor
+from oracle.planeval import evaluate
+from tests.helpers import assertSameResult, toleranceColumns
+
+pytestmark = pytest.mark.gpu
+
+_MODULES = {}
+_STORE = {}
+
+
+def module(name):
+    if name not in _MODULES:
+        comp = planlib.translate(name)
+        _MODULES[name] = rt.PlanModule(comp.compile(name))
+    return _MODULES[name]
+
+
+def runPlan(name, acc, tuning=None, storeKey=None):
+    """GPU result (decoded) and the resolved plan used to compute it."""
+    mod = module(name)
+    store = _STORE.setdefault(storeKey, {}) if storeKey else {}
+    inputs = rt.BoundInputs(mod, acc, store=store)
+    ex = rt.run(mod, inputs, tuning or rt.newTuning())
+    res = runner.resultColumns(mod, ex, acc)
+    retries = ex.res.retries
+    ex.release()
+    return res, retries
+
+
+def oracleFor(name, acc):
+    alg, p, cfg, opts = planlib.build(name, acc)
+    plan = alg.resolveAlgebraPlan(p, cfg)
+    return evaluate(plan, acc), plan
+
+
+def check(name, acc, tuning=None, storeKey=None, label=None):
+    print("gpu case: %s %s" % (name, label or ""), flush=True)
+    got, retries = runPlan(name, acc, tuning, storeKey)
+    want, plan = oracleFor(name, acc)
+    assertSameResult(got, want, toleranceColumns(plan), label="%s %s" % (name, label or ""))
+    return got, retries
+
+
+def workloadAcc(config, rows=None, seed=None):
+    return MemoryAccessor(workloads.generate(config, rows, seed))
+
+
+# ------------------------------------------------------------ workloads
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1000, 100003])
+def test_c0_ragged(n):
+    check("c0", workloadAcc("c0", {"cfg0": n}, seed=11 + n), label="n=%d" % n)
+
+
+def test_c0_full():
+    check("c0", workloadAcc("c0"), storeKey="c0full", label="1M rows")
+
+
+@pytest.mark.parametrize("lds", [(-1, 0), (2, 0), (16, 1), (16, 8), (4096, 0)])
+def test_c0_lds_tiers(lds):
+    t = rt.newTuning()
+    for i in range(rt.MAX_PIPES):
+        t.lds_slots[i], t.lds_replicas[i] = lds
+    check("c0", workloadAcc("c0", {"cfg0": 200000}, seed=5), tuning=t, label="lds=%r" % (lds,))
+
+
+@pytest.mark.parametrize("grid", [1, 7, 4096])
+def test_c0_grid(grid):
+    t = rt.newTuning()
+    for i in range(rt.MAX_PIPES):
+        t.grid[i] = grid
+    check("c0", workloadAcc("c0", {"cfg0": 300000}, seed=6), tuning=t, label="grid=%d" % grid)
+
+
+def test_c1_full():
+    check("c1", workloadAcc("c1"), storeKey="c1full", label="1.5M x 6M")
+
+
+@pytest.mark.parametrize("rows", [{"dim": 0, "fact": 1000}, {"dim": 1000, "fact": 0}, {"dim": 1, "fact": 1},
+                                  {"dim": 3000, "fact": 20000}])
+def test_c1_small(rows):
+    check("c1", workloadAcc("c1", rows, seed=21), label=repr(rows))
+
+
+def test_c1_undersized_tables_retry():
+    t = rt.newTuning()
+    t.ht_capacity[0] = 16   # join table
+    t.ht_capacity[1] = 8    # group table
+    _, retries = check("c1", workloadAcc("c1", {"dim": 5000, "fact": 40000}, seed=22), tuning=t,
+                       label="tiny tables")
+    assert retries > 0
+
+
+@pytest.mark.parametrize("variant", ["c2_none", "c2_refill", "c2_pushdown", "c2_both"])
+def test_c2_small(variant):
+    check(variant, workloadAcc("c2", {"build": 2000, "probe": 50000}, seed=31), label="small")
+
+
+@pytest.mark.parametrize("variant", ["c2_none", "c2_refill", "c2_pushdown", "c2_both"])
+def test_c2_empty(variant):
+    check(variant, workloadAcc("c2", {"build": 0, "probe": 100}, seed=32), label="empty build")
+    check(variant, workloadAcc("c2", {"build": 100, "probe": 0}, seed=33), label="empty probe")
+
+
+@pytest.mark.parametrize("thr", [1, 17, 64])
+def test_c2_refill_thresholds(thr):
+    t = rt.newTuning()
+    for i in range(rt.MAX_PIPES):
+        t.refill_threshold[i] = thr
+    check("c2_both", workloadAcc("c2", {"build": 5000, "probe": 100000}, seed=34), tuning=t, label="thr=%d" % thr)
+
+
+def c2ClosedForm(acc):
+    """COUNT and SUM(b_pay * p_pay) of c2 without enumerating pairs:
+    per key k, count = nb(k) * np(k) and sum = Sb(k) * Sp(k)."""
+    bk = np.asarray(acc.column("build", "b_key")).astype(np.int64)
+    bp = np.asarray(acc.column("build", "b_pay"))
+    pk = np.asarray(acc.column("probe", "p_key")).astype(np.int64)
+    pf = np.asarray(acc.column("probe", "p_f"))
+    pp = np.asarray(acc.column("probe", "p_pay"))
+    m = pf < 10
+    pk, pp = pk[m], pp[m]
+    K = int(max(bk.max(initial=0), pk.max(initial=0))) + 1
+    nb = np.bincount(bk, minlength=K).astype(object)
+    npr = np.bincount(pk, minlength=K).astype(object)
+    sb = np.zeros(K, dtype=object)
+    sp = np.zeros(K, dtype=object)
+    np.add.at(sb, bk, bp.astype(object))
+    np.add.at(sp, pk, pp.astype(object))
+    cnt = int(np.dot(nb, npr))
+    s = int(np.dot(sb, sp))
+    return cnt, (s + 2 ** 63) % 2 ** 64 - 2 ** 63
+
+
+@pytest.mark.parametrize("variant", ["c2_none", "c2_refill", "c2_pushdown", "c2_both"])
+def test_c2_full_closed_form(variant):
+    acc = _STORE.setdefault("c2acc", workloadAcc("c2"))
+    print("gpu case: %s full 1M x 16M" % variant, flush=True)
+    got, _ = runPlan(variant, acc, storeKey="c2full")
+    cnt, s = c2ClosedForm(acc)
+    assert int(got["cnt"][0]) == cnt
+    assert int(got["sum_prod"][0]) == s
+
+
+# ------------------------------------------------------------ features
+
+@pytest.mark.parametrize("name", [n for n in planlib.PLANS if planlib.PLANS[n][0] == "feature"])
+@pytest.mark.parametrize("size", [(5000, 700), (200, 50), (1, 1), (0, 0)])
+def test_feature_plans(name, size):
+    acc = planlib.featureData(*size)
+    check(name, acc, label="t=%d u=%d" % size)
+
+
+def test_feature_retry_all_buffers():
+    acc = planlib.featureData()
+    t = rt.newTuning()
+    t.result_capacity = 1
+    t.mm_entries[0] = 1
+    _, retries = check("f_join_multi", acc, tuning=t, label="tiny result + entries")
+    assert retries > 0
+    t = rt.newTuning()
+    t.ht_capacity[0] = 2
+    _, retries = check("f_groupby_wide", acc, tuning=t, label="tiny group table")
+    assert retries > 0
