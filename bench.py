@@ -1,0 +1,216 @@
+"""Benchmark: the filtered PK-FK join + aggregation (dim 1.5M x fact 6M) on
+one MI355X, through the C ABI, with tables resident in HBM.
+
+A step is one full execution of the plan: the workspace init kernel, the dim
+build kernel, the fact probe/aggregate kernel and the group-table scan
+kernel, enqueued on one stream.  ``value`` is fact (probe-side) rows per
+second of the whole job.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0]
+                  [--rows-scale S] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "probe rows/s + HBM GB/s vs peak; batched vs serial queries/s and p99 at fixed QPS"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per row of the dominant (scan/probe) kernel: every column
+# the plan references on that table, read once
+WORKLOADS = {
+    "c1": {"table": "fact", "kernel": "krnl_fact3", "bytes_per_row": 4 + 8 + 8 + 4,
+           "desc": "filtered PK-FK hash join + group-by: dim 1.5M rows (d_cat == '2'), fact 6M rows "
+                   "(1-year date window), group by d_key, SUM f_price*(1-f_disc); int32 keys, f64 sums"},
+    "c0": {"table": "cfg0", "kernel": "krnl_cfg01", "bytes_per_row": 1 + 1 + 4 + 8 + 8 + 8 + 4,
+           "desc": "scan-filter-groupby (Q1 shape): date <= cutoff, group by (g1, g2), "
+                   "5 f64 sums + count"},
+    "cal": {"table": "k", "kernel": "krnl_k1", "bytes_per_row": 8 + 8,
+            "desc": "copy of one int64 column (8 B read + 8 B written per row): counter calibration"},
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--rows-scale", type=float, default=1.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    return ap.parse_args()
+
+
+def cpuBaseline(workload, g, seconds):
+    """The C oracle (oracle/cpu_oracle.c) on all host threads, repeated on the
+    full workload for about `seconds`; median time -> rows/s."""
+    from oracle import coracle
+    threads = os.cpu_count() or 1
+    cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
+    if workload == "c1":
+        run = lambda: coracle.c1(cols["dim"], cols["fact"], threads=threads)
+        rows = len(cols["fact"]["f_fk"])
+    else:
+        run = lambda: coracle.c0(cols["cfg0"], threads=threads)
+        rows = len(cols["cfg0"]["qty"])
+    run()
+    times = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": rows / med, "unit": "rows/s", "cores": threads, "kind": "port",
+            "sample": "full workload (%d probe rows), %d repetitions in %.1f s, median %.2f ms"
+                      % (rows, len(times), sum(times), med * 1e3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from dogqc_amd import planlib, workloads, runtime as rt, runner
+    from dogqc_amd.dbio import MemoryAccessor
+
+    spec = WORKLOADS[args.workload]
+    rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[args.workload].items()}
+    g = workloads.generate(args.workload, rows)
+    acc = MemoryAccessor(g)
+    comp = planlib.translate(args.workload)
+    mod = rt.PlanModule(comp.compile(args.workload))
+    inputs = rt.BoundInputs(mod, acc)
+    tuning = rt.newTuning()
+
+    # first run sizes the tables (retries update `tuning`) and checks parity
+    ex = rt.run(mod, inputs, tuning)
+    got = runner.resultColumns(mod, ex, acc)
+    ex.release()
+    parity = checkParity(args.workload, g, got)
+    if args.workload == "cal":
+        args.no_cpu = True
+
+    stream = rt.Stream()
+    lay = rt.layout(mod, inputs, tuning)
+    ws = rt.DeviceBuffer(lay.ws_bytes)
+    steps = mod.steps()
+    for _ in range(args.warmup):
+        rt.enqueue(mod, inputs, tuning, ws, stream)
+    stream.sync()
+
+    events = [[rt.Event() for _ in range(len(steps) + 1)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rt.enqueue(mod, inputs, tuning, ws, stream, events[k])
+    stream.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    per = {name: [] for name in steps}
+    for k in range(args.steps):
+        for i, name in enumerate(steps):
+            per[name].append(events[k][i].elapsed(events[k][i + 1]))
+    kernel_ms = {name: statistics.mean(v) for name, v in per.items()}
+
+    # the same plan captured once into a hipGraph and replayed
+    graph_ms = graphReplay(mod, inputs, tuning, ws, stream, args.steps)
+
+    probe_rows = rows[spec["table"]]
+    dom = spec["kernel"]
+    dom_ms = kernel_ms[dom]
+    alg_bytes = spec["bytes_per_row"] * probe_rows
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = loadTraffic(args.workload, dom)
+
+    out = {
+        "metric": METRIC,
+        "value": probe_rows / (ms_per_step * 1e-3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded PCG64 generator, dogqc_amd/workloads.py)",
+        "config": {"workload": spec["desc"], "rows": rows, "seed": workloads.SEEDS[args.workload],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                     "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
+        "kernels_ms": kernel_ms,
+        "graph_ms_per_step": graph_ms,
+        "parity": parity,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpuBaseline(args.workload, g, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+def graphReplay(mod, inputs, tuning, ws, stream, steps):
+    import ctypes as C
+    from dogqc_amd import runtime as rt
+    L = rt.lib()
+    gx = C.c_void_p()
+    rt.check(L.dqc_graph_create(mod.handle, C.byref(inputs.struct), C.byref(tuning), ws.ptr, ws.nbytes,
+                                stream.handle, C.byref(gx)), "dqc_graph_create")
+    for _ in range(3):
+        rt.check(L.dqc_graph_launch(gx, stream.handle), "dqc_graph_launch")
+    stream.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rt.check(L.dqc_graph_launch(gx, stream.handle), "dqc_graph_launch")
+    stream.sync()
+    t1 = time.perf_counter()
+    rt.check(L.dqc_graph_destroy(gx), "dqc_graph_destroy")
+    return (t1 - t0) * 1e3 / steps
+
+
+def checkParity(workload, g, got):
+    """Bit-exact keys/counts, f64 sums within 1e-9 relative, against the C oracle."""
+    import numpy as np
+    from oracle import coracle
+    cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
+    if workload == "cal":
+        ok = np.array_equal(np.sort(got["k_v"]), cols["k"]["k_v"])
+        return {"oracle": "identity", "rows": int(len(got["k_v"])), "ok": bool(ok)}
+    if workload == "c1":
+        keys, sums = coracle.c1(cols["dim"], cols["fact"])
+        o = np.argsort(keys)
+        gk = np.asarray(got["d_key"])
+        go = np.argsort(gk)
+        ok = np.array_equal(gk[go], keys[o]) and np.allclose(got["revenue"][go], sums[o], rtol=1e-9, atol=0)
+        return {"oracle": "C", "groups": int(len(keys)), "ok": bool(ok)}
+    ref = coracle.c0(cols["cfg0"])
+    ok = len(ref) == len(got["g1"])
+    for i in range(len(got["g1"])):
+        r = ref.get((int(got["g1"][i]), int(got["g2"][i])))
+        ok = ok and r is not None and int(got["count_order"][i]) == r[5] and \
+            np.allclose([got["sum_qty"][i], got["sum_base_price"][i], got["sum_disc_price"][i],
+                         got["sum_charge"][i]], r[:4], rtol=1e-9, atol=0)
+    return {"oracle": "C", "groups": len(ref), "ok": bool(ok)}
+
+
+def loadTraffic(workload, kernel):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 PMC summary (profiles/<kernel>_<workload>_traffic.json)."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+if __name__ == "__main__":
+    main()

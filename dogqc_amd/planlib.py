@@ -162,6 +162,7 @@ PLANS["c2_none"] = ("c2", {})
 PLANS["c2_refill"] = ("c2", {"buffers": [3]})
 PLANS["c2_pushdown"] = ("c2", {"pushdown": True})
 PLANS["c2_both"] = ("c2", {"buffers": [3], "pushdown": True})
+PLANS["cal"] = ("cal", {})
 for _n in FEATURE_PLANS:
     PLANS[_n] = ("feature", {})
 PLANS["f_join_multi_pushdown"] = ("feature", {"pushdown": True, "base": "f_join_multi"})

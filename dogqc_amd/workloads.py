@@ -15,6 +15,8 @@ in the reference plan API.
       values, b_pay INT64 U[1, 2^20)), probe (p_key INT U[0, 200k),
       p_f INT U[0,100), p_pay INT64 U[1, 2^20)); filter p_f < 10;
       COUNT and SUM b_pay * p_pay.
+  cal a one-column INT64 table copied to the result (8 bytes read and 8
+      written per row): calibrates memory-traffic counters.
 
 Random numbers come from numpy's PCG64 generator, so a (config, rows, seed)
 triple always yields the same columns.
@@ -28,10 +30,11 @@ from dogqc_amd.relationalAlgebra import RelationalAlgebra, Reduction
 from dogqc_amd.types import Type
 from dogqc_amd.dbio import MemoryAccessor
 
-SEEDS = {"c0": 1000, "c1": 1001, "c2": 1002, "c3": 1003}
+SEEDS = {"c0": 1000, "c1": 1001, "c2": 1002, "c3": 1003, "cal": 0}
 FULL_ROWS = {"c0": {"cfg0": 1000000},
              "c1": {"dim": 1500000, "fact": 6000000},
-             "c2": {"build": 1000000, "probe": 16000000}}
+             "c2": {"build": 1000000, "probe": 16000000},
+             "cal": {"k": 64 * 1024 * 1024}}
 
 
 def dateTable():
@@ -75,6 +78,8 @@ def schema(config):
         return {"dim": OrderedDict([("d_key", Type.INT), ("d_cat", Type.CHAR), ("d_seg", Type.INT)]),
                 "fact": OrderedDict([("f_fk", Type.INT), ("f_price", Type.DOUBLE), ("f_disc", Type.DOUBLE),
                                      ("f_date", Type.DATE)])}
+    if config == "cal":
+        return {"k": OrderedDict([("k_v", Type.INT64)])}
     if config == "c2":
         return {"build": OrderedDict([("b_key", Type.INT), ("b_pay", Type.INT64)]),
                 "probe": OrderedDict([("p_key", Type.INT), ("p_f", Type.INT), ("p_pay", Type.INT64)])}
@@ -131,6 +136,8 @@ def generate(config, rows=None, seed=None):
         probe["p_f"] = rng.integers(0, 100, size=np_).astype(np.int32)
         probe["p_pay"] = rng.integers(1, 1 << 20, size=np_, dtype=np.int64)
         out["build"], out["probe"] = build, probe
+    elif config == "cal":
+        out["k"] = OrderedDict([("k_v", np.arange(rows["k"], dtype=np.int64))])
     else:
         raise KeyError(config)
     typed = OrderedDict()
@@ -202,6 +209,10 @@ def plan(config, acc):
                              alg.selection(scal.SmallerExpr(scal.AttrExpr("p_f"), scal.ConstExpr("10", Type.INT)),
                                            alg.scan("probe")))))
         cfg = {3: {"selectivity": 0.1}, 4: {"multimatch": True, "selectivity": 0.5}}
+    elif config == "cal":
+        # copy of one 8-byte column: reads and writes exactly 8 bytes per row
+        p = alg.projection(["k_v"], alg.scan("k"))
+        cfg = {}
     else:
         raise KeyError(config)
     return alg, p, cfg
