@@ -115,7 +115,7 @@ class EquiJoinTranslator(BinaryTranslator):
         idx = ht["index"]
         prog.hostVars["htcap%d" % idx] = (
             "L.htcap[%d] = t->ht_capacity[%d] > 0 ? t->ht_capacity[%d] : "
-            "std::max<int64_t>(64, (int64_t)((double)%s * %r * fac(t->ht_factor[%d])));\n"
+            "std::max<int64_t>(1, (int64_t)((double)%s * %r * fac(t->ht_factor[%d])));\n"
             "if (L.htcap[%d] >= (int64_t)0xffffffffll) return DQC_ENOMEM;"
             % (idx, idx, idx, est, float(a.htSizeFactor), idx, idx))
         self.storedBuffers = []
@@ -142,33 +142,34 @@ class EquiJoinTranslator(BinaryTranslator):
             b = prog.buffer("jht%d_p%d" % (J, k), ct, count)
             self.storedBuffers.append((b, ct))
             fields.append(pipe.param("jht%d_p%d" % (J, k), "%s*" % ct, "B_" + b))
-        pipe.code.open("if (active)")
-        self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
-        if kind == "unique":
-            pipe.emit("const int64_t js = dqc::ht_insert(%s, %s, jtag%d, jh%d);" % (ftags, fcap, J, J))
-            pipe.code.open("if (js >= 0)")
-            for f, att in zip(fields, self.stored):
-                pipe.emit("%s[js] = %s;" % (f, pipe.access(att)))
+        with pipe.rowLoop():
+            pipe.code.open("if (active[u])")
+            self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
+            if kind == "unique":
+                pipe.emit("const int64_t js = dqc::ht_insert(%s, %s, jtag%d, jh%d);" % (ftags, fcap, J, J))
+                pipe.code.open("if (js >= 0)")
+                for f, att in zip(fields, self.stored):
+                    pipe.emit("%s[js] = %s;" % (f, pipe.access(att)))
+                pipe.code.close()
+                pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
+                          "atomicOr(&st->ht_overflow_mask, %du); }" % (1 << idx))
+            else:
+                pipe.phased = True
+                pipe.phasedHt = ht
+                pipe.emit("const uint32_t jb = dqc::slot_of(jh%d, %s);" % (J, fcap))
+                pipe.code.open("if constexpr (PHASE == 0)")
+                pipe.emit("atomicAdd(&%s[2u * jb + 1u], 1u);" % fdir)
+                pipe.code.close()
+                pipe.code.open("else")
+                pipe.emit("const uint32_t jpos = %s[2u * jb] + atomicAdd(&%s[2u * jb + 1u], 1u);" % (fdir, fdir))
+                pipe.code.open("if (jpos < %s)" % fecap)
+                for f, att in zip(fields, self.stored):
+                    pipe.emit("%s[jpos] = %s;" % (f, pipe.access(att)))
+                pipe.code.close()
+                pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); "
+                          "atomicOr(&st->mm_overflow_mask, %du); }" % (1 << idx))
+                pipe.code.close()
             pipe.code.close()
-            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
-                      % (1 << idx))
-        else:
-            pipe.phased = True
-            pipe.phasedHt = ht
-            pipe.emit("const uint32_t jb = dqc::slot_of(jh%d, %s);" % (J, fcap))
-            pipe.code.open("if constexpr (PHASE == 0)")
-            pipe.emit("atomicAdd(&%s[2u * jb + 1u], 1u);" % fdir)
-            pipe.code.close()
-            pipe.code.open("else")
-            pipe.emit("const uint32_t jpos = %s[2u * jb] + atomicAdd(&%s[2u * jb + 1u], 1u);" % (fdir, fdir))
-            pipe.code.open("if (jpos < %s)" % fecap)
-            for f, att in zip(fields, self.stored):
-                pipe.emit("%s[jpos] = %s;" % (f, pipe.access(att)))
-            pipe.code.close()
-            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); atomicOr(&st->mm_overflow_mask, %du); }"
-                      % (1 << idx))
-            pipe.code.close()
-        pipe.code.close()
 
     # ----------------------------------------------------------- probe
     def _storedFields(self, pipe):
@@ -179,23 +180,25 @@ class EquiJoinTranslator(BinaryTranslator):
         return out
 
     def _bindBuild(self, pipe, fields, indexReg):
-        """Build-side attributes become lazy loads from the stored columns."""
+        """Build-side attributes become lazy loads from the stored columns at
+        the row's position ``indexReg[u]``."""
         for f, att, (b, ct) in zip(fields, self.stored, self.storedBuffers):
-            pipe.setLazy(att, "%s[%s]" % (f, indexReg), [indexReg], ct)
+            pipe.setLazy(att, "%s[%s[u]]" % (f, indexReg), [indexReg], ct)
 
     def _matchCond(self, ctxt, pipe, fields, indexReg):
-        """C expression: stored keys at indexReg equal the probe keys and the
-        extra join conditions hold.  Loads what it reads explicitly."""
+        """C expression: stored keys at indexReg[u] equal the probe keys and
+        the extra join conditions hold.  Loads what it reads explicitly."""
         a = self.algExpr
+        ix = indexReg + "[u]"
         parts = []
         if a.multimatch or not self.exact:
             byId = {att.id: f for f, att in zip(fields, self.stored)}
             for b, p in zip(self.buildKeys, self.probeKeys):
                 f = byId[b.id]
                 if b.dataType == Type.STRING:
-                    parts.append("dqc::string_equals(%s[%s], %s)" % (f, indexReg, pipe.access(p)))
+                    parts.append("dqc::string_equals(%s[%s], %s)" % (f, ix, pipe.access(p)))
                 else:
-                    parts.append("(%s[%s] == %s)" % (f, indexReg, pipe.access(p)))
+                    parts.append("(%s[%s] == %s)" % (f, ix, pipe.access(p)))
         if a.conditions is not None:
             buildIds = set(att.id for att in self.stored)
             condAttrs = []
@@ -204,7 +207,7 @@ class EquiJoinTranslator(BinaryTranslator):
             for att in condAttrs:
                 st = pipe.attrs.get(att.id)
                 if att.id in buildIds:
-                    pipe.emit("%s = %s;" % (st.reg, st.loader))
+                    pipe.emit("%s[u] = %s;" % (st.reg, st.loader))
                     st.loaded = True
             parts.append("(%s)" % ctxt.exprEmitter().emit(a.conditions))
         return " && ".join(parts) if parts else "true"
@@ -223,34 +226,34 @@ class EquiJoinTranslator(BinaryTranslator):
             ftags = pipe.param("jht%d" % J, "const uint64_t*", "B_jht%d_tags" % J)
             fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
             fields = self._storedFields(pipe)
-            slot = pipe.reg("jslot%d" % J, "int64_t", "-1")
-            pipe.indexRegs[slot] = "int64_t"
-            self._bindBuild(pipe, fields, slot)
-            pipe.emit("%s = -1;" % slot)
-            pipe.code.open("if (active)")
-            self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
-            pipe.emit("uint32_t jc = dqc::probe_start(jh%d, %s), jv = 0;" % (J, fcap))
-            needCheck = (not self.exact) or a.conditions is not None
-            if needCheck:
-                pipe.code.open("for (;;)")
-                pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
-                pipe.emit("if (%s < 0) break;" % slot)
-                pipe.emit("if (%s) break;" % self._matchCond(ctxt, pipe, fields, slot))
+            slotName = "jslot%d" % J
+            slot = pipe.reg(slotName, "int64_t", "-1")
+            pipe.indexRegs[slotName] = "int64_t"
+            self._bindBuild(pipe, fields, slotName)
+            with pipe.rowLoop():
+                pipe.emit("%s = -1;" % slot)
+                pipe.code.open("if (active[u])")
+                self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+                pipe.emit("uint32_t jc = dqc::probe_start(jh%d, %s), jv = 0;" % (J, fcap))
+                needCheck = (not self.exact) or a.conditions is not None
+                if needCheck:
+                    pipe.code.open("for (;;)")
+                    pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
+                    pipe.emit("if (%s < 0) break;" % slot)
+                    pipe.emit("if (%s) break;" % self._matchCond(ctxt, pipe, fields, slotName))
+                    pipe.code.close()
+                else:
+                    pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
+                if a.joinType == Join.ANTI:
+                    pipe.emit("active[u] = %s < 0;" % slot)
+                else:
+                    pipe.emit("active[u] = %s >= 0;" % slot)
                 pipe.code.close()
-            else:
-                pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
-            if a.joinType == Join.INNER:
-                pipe.emit("active = %s >= 0;" % slot)
-            elif a.joinType == Join.SEMI:
-                pipe.emit("active = %s >= 0;" % slot)
-            elif a.joinType == Join.ANTI:
-                pipe.emit("active = %s < 0;" % slot)
-            pipe.code.close()
-            if a.joinType == Join.INNER:
-                for b in self.keyReconstruct:
-                    p = self.probeKeys[self.buildKeys.index(b)]
-                    st = pipe.setComputed(b)
-                    pipe.emit("if (active) %s = %s;" % (st.reg, pipe.access(p)))
+                if a.joinType == Join.INNER:
+                    for b in self.keyReconstruct:
+                        p = self.probeKeys[self.buildKeys.index(b)]
+                        st = pipe.setComputed(b)
+                        pipe.emit("if (active[u]) %s[u] = %s;" % (st.reg, pipe.access(p)))
             self.parent.consume(ctxt)
             return
 
@@ -260,51 +263,57 @@ class EquiJoinTranslator(BinaryTranslator):
         # flagged an overflow and the run is repeated): clamp every range
         self.fecap = pipe.param("jht%d_ecap" % J, "uint32_t", "L.mment[%d]" % idx)
         fields = self._storedFields(pipe)
-        moff = pipe.reg("moff%d" % J, "uint32_t", "0")
-        pos = pipe.reg("mpos%d" % J, "uint32_t", "0")
-        pipe.indexRegs[pos] = "uint32_t"
+        posName = "mpos%d" % J
+        pos = pipe.reg(posName, "uint32_t", "0")
+        pipe.indexRegs[posName] = "uint32_t"
         if a.joinType in (Join.SEMI, Join.ANTI):
-            pipe.code.open("if (active)")
-            self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
-            pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
-            pipe.emit("int fm = 0;")
-            pipe.emit("const uint32_t jend = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);" % self.fecap)
-            pipe.code.open("for (%s = jd.x; %s < jend; ++%s)" % (pos, pos, pos))
-            self._bindBuild(pipe, fields, pos)
-            pipe.emit("if (%s) { fm = 1; break; }" % self._matchCond(ctxt, pipe, fields, pos))
-            pipe.code.close()
-            pipe.emit("active = %s;" % ("fm" if a.joinType == Join.SEMI else "!fm"))
-            pipe.code.close()
+            with pipe.rowLoop():
+                pipe.code.open("if (active[u])")
+                self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+                pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
+                pipe.emit("int fm = 0;")
+                pipe.emit("const uint32_t jend = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);"
+                          % self.fecap)
+                pipe.code.open("for (%s = jd.x; %s < jend; ++%s)" % (pos, pos, pos))
+                self._bindBuild(pipe, fields, posName)
+                pipe.emit("if (%s) { fm = 1; break; }" % self._matchCond(ctxt, pipe, fields, posName))
+                pipe.code.close()
+                pipe.emit("active[u] = %s;" % ("fm" if a.joinType == Join.SEMI else "!fm"))
+                pipe.code.close()
             self.parent.consume(ctxt)
             return
         if a.joinType == Join.OUTER:
             raise NotImplementedError("outer joins are not supported by the HIP backend yet")
-
+        if pipe.U != 1:
+            raise RuntimeError("multimatch probes need a one-row-per-thread pipeline")
         probeSide = self.rightChild.algExpr.outRelation
         probeLive = [att for id, att in a.outRelation.items() if id in probeSide]
         if EquiJoinTranslator.usePushDownJoin and ctxt.innerLoopCount == 0:
-            self._probePushDown(ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive)
+            self._probePushDown(ctxt, pipe, fdir, fcap, fields, posName, probeLive)
         else:
-            self._probeLoop(ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive)
+            self._probeLoop(ctxt, pipe, fdir, fcap, fields, posName, probeLive)
 
-    def _probeLoop(self, ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive):
+    def _probeLoop(self, ctxt, pipe, fdir, fcap, fields, posName, probeLive):
         """Each lane walks its own match list; the wavefront loops while any
         lane has matches left (hashJoins.py:465-534)."""
         J = self.algExpr.opId
+        moff = pipe.reg("moff%d" % J, "uint32_t", "0")
         mend = pipe.reg("mend%d" % J, "uint32_t", "0")
+        pos = posName + "[u]"
         pipe.emit("%s = 0; %s = 0;" % (moff, mend))
-        pipe.code.open("if (active)")
+        pipe.code.open("if (active[u])")
         self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
         pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
-        pipe.emit("%s = jd.x; %s = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);" % (moff, mend, self.fecap))
+        pipe.emit("%s = jd.x; %s = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);"
+                  % (moff, mend, self.fecap))
         pipe.code.close()
         pipe.need(probeLive)
-        pipe.emit("int mact%d = active && %s < %s;" % (J, moff, mend))
+        pipe.emit("int mact%d = active[u] && %s < %s;" % (J, moff, mend))
         pipe.code.open("while (dqc::ballot(mact%d))" % J)
         pipe.emit("%s = %s;" % (pos, moff))
-        self._bindBuild(pipe, fields, pos)
-        pipe.emit("active = mact%d;" % J)
-        pipe.emit("if (active) active = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, pos))
+        self._bindBuild(pipe, fields, posName)
+        pipe.emit("active[u] = mact%d;" % J)
+        pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, posName))
         ctxt.innerLoopCount += 1
         pipe.innerDepth += 1
         self.parent.consume(ctxt)
@@ -313,20 +322,23 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.emit("++%s;" % moff)
         pipe.emit("mact%d = mact%d && %s < %s;" % (J, J, moff, mend))
         pipe.code.close()
-        pipe.emit("active = 0;")
+        pipe.emit("active[u] = 0;")
 
-    def _probePushDown(self, ctxt, pipe, fdir, fcap, fields, moff, pos, probeLive):
+    def _probePushDown(self, ctxt, pipe, fdir, fcap, fields, posName, probeLive):
         """Push-down parallelism for wave64 (see module docstring)."""
         J = self.algExpr.opId
         nw = pipe.bs // 64
         pipe.staticLds.append("__shared__ uint64_t pdincl%d[%d][64];" % (J, nw))
         pipe.annotations.append("PD%d" % J)
+        moff = pipe.reg("moff%d" % J, "uint32_t", "0")
         mlen = pipe.reg("mlen%d" % J, "uint64_t", "0")
+        pos = posName + "[u]"
         pipe.emit("%s = 0; %s = 0;" % (moff, mlen))
-        pipe.code.open("if (active)")
+        pipe.code.open("if (active[u])")
         self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
         pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
-        pipe.emit("%s = jd.x; %s = jd.x < %s ? std::min<uint64_t>(jd.y, %s - jd.x) : 0;" % (moff, mlen, self.fecap, self.fecap))
+        pipe.emit("%s = jd.x; %s = jd.x < %s ? std::min<uint64_t>(jd.y, %s - jd.x) : 0;"
+                  % (moff, mlen, self.fecap, self.fecap))
         pipe.code.close()
         pipe.need(probeLive)
         # everything the matching and the operators above read from the probe
@@ -338,15 +350,15 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.emit("pdincl%d[wave][lane] = pincl%d;" % (J, J))
         saved = []
         for r, ct in live.items():
-            s = pipe.reg("%s_pd%d" % (r, J), ct)
-            pipe.emit("%s = %s;" % (s, r))
-            saved.append((r, s))
+            sv = pipe.reg("%s_pd%d" % (r, J), ct)
+            pipe.emit("%s = %s[u];" % (sv, r))
+            saved.append((r, sv))
         pipe.emit("const uint32_t moffs%d = %s;" % (J, moff))
         pipe.code.open("for (uint64_t pdone = 0; pdone < ptotal%d; pdone += 64)" % J)
         pipe.emit("const uint64_t pt = pdone + lane;")
-        pipe.emit("active = pt < ptotal%d;" % J)
+        pipe.emit("active[u] = pt < ptotal%d;" % J)
         pipe.emit("int po = 0;")
-        pipe.code.open("if (active)")
+        pipe.code.open("if (active[u])")
         pipe.emit("int lo = 0, hi = 63;")
         pipe.emit("while (lo < hi) { const int mid = (lo + hi) >> 1; "
                   "if (pdincl%d[wave][mid] > pt) hi = mid; else lo = mid + 1; }" % J)
@@ -354,14 +366,14 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.code.close()
         pipe.emit("const uint64_t pbase = po ? pdincl%d[wave][po - 1] : 0;" % J)
         pipe.emit("%s = dqc::shfl(moffs%d, po) + (uint32_t)(pt - pbase);" % (pos, J))
-        for r, s in saved:
-            pipe.emit("%s = dqc::shfl(%s, po);" % (r, s))
-        self._bindBuild(pipe, fields, pos)
-        pipe.emit("if (active) active = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, pos))
+        for r, sv in saved:
+            pipe.emit("%s[u] = dqc::shfl(%s, po);" % (r, sv))
+        self._bindBuild(pipe, fields, posName)
+        pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, posName))
         ctxt.innerLoopCount += 1
         pipe.innerDepth += 1
         self.parent.consume(ctxt)
         pipe.innerDepth -= 1
         ctxt.innerLoopCount -= 1
         pipe.code.close()
-        pipe.emit("active = 0;")
+        pipe.emit("active[u] = 0;")

@@ -35,7 +35,7 @@ class LaneRefill(UnaryTranslator):
     def consume(self, ctxt):
         pipe = ctxt.pipe
         X = self.algExpr.opId
-        if ctxt.innerLoopCount > 0 or pipe.innerDepth > 0:
+        if ctxt.innerLoopCount > 0 or pipe.innerDepth > 0 or pipe.U != 1:
             pipe.emit("// lane refill at opId %d skipped inside an inner loop" % X)
             self.parent.consume(ctxt)
             return
@@ -49,7 +49,7 @@ class LaneRefill(UnaryTranslator):
             q = "rq%d_%s" % (X, r)
             pipe.staticLds.append("__shared__ %s %s[%d][128];" % (ct, q, nw))
             queues.append((r, q))
-        cnt = pipe.reg("rcnt%d" % X, "uint32_t", "0")
+        cnt = pipe.reg("rcnt%d" % X, "uint32_t", "0", perRow=False)
         default = max(1, min(64, int(64 * self.threshold)))
         pid = pipe.pipeId
         thr = pipe.param("rthr%d" % X, "uint32_t",
@@ -58,11 +58,11 @@ class LaneRefill(UnaryTranslator):
         outer = pipe.refillPoints[-1] if pipe.refillPoints else None
         pipe.emit("// lane refill (opId %d)" % X)
         pipe.code.open("")
-        pipe.emit("const uint64_t rm = dqc::ballot(active);")
+        pipe.emit("const uint64_t rm = dqc::ballot(active[u]);")
         pipe.emit("const uint32_t rpos = %s + dqc::lanes_below(rm);" % cnt)
-        pipe.code.open("if (active)")
+        pipe.code.open("if (active[u])")
         for r, q in queues:
-            pipe.emit("%s[wave][rpos] = %s;" % (q, r))
+            pipe.emit("%s[wave][rpos] = %s[u];" % (q, r))
         pipe.code.close()
         pipe.emit("%s += dqc::popc64(rm);" % cnt)
         pipe.code.close()
@@ -76,15 +76,15 @@ class LaneRefill(UnaryTranslator):
         pipe.refillPoints.append(point)
         pipe.emit("const uint32_t rtake = %s < 64u ? %s : 64u;" % (cnt, cnt))
         pipe.emit("%s -= rtake;" % cnt)
-        pipe.emit("active = lane < rtake;")
-        pipe.code.open("if (active)")
+        pipe.emit("active[u] = lane < rtake;")
+        pipe.code.open("if (active[u])")
         for r, q in queues:
-            pipe.emit("%s = %s[wave][%s + lane];" % (r, q, cnt))
+            pipe.emit("%s[u] = %s[wave][%s + lane];" % (r, q, cnt))
         pipe.code.close()
         point["cond"] = "%s >= (rflush%d ? 1u : %s)" % (cnt, X, thr)
         self.parent.consume(ctxt)
         pipe.code.close()
-        pipe.emit("active = 0;")
+        pipe.emit("active[u] = 0;")
 
 
 def finishRefills(pipe):

@@ -30,8 +30,9 @@ class LaneActivityProfiler(UnaryTranslator):
         f = pipe.param("prof", "unsigned long long*", "(unsigned long long*)B_profile")
         on = pipe.param("prof_on", "int32_t", "t->profile")
         pipe.emit("// lane activity profile (opId %d)" % P)
-        pipe.code.open("if (%s)" % on)
-        pipe.emit("const uint64_t pm = dqc::ballot(active);")
-        pipe.emit("if (lane == 0) atomicAdd(&%s[%d * DQC_PROFILE_BUCKETS + dqc::popc64(pm)], 1ull);" % (f, slot))
-        pipe.code.close()
+        with pipe.rowLoop():
+            pipe.code.open("if (%s)" % on)
+            pipe.emit("const uint64_t pm = dqc::ballot(active[u]);")
+            pipe.emit("if (lane == 0) atomicAdd(&%s[%d * DQC_PROFILE_BUCKETS + dqc::popc64(pm)], 1ull);" % (f, slot))
+            pipe.code.close()
         self.parent.consume(ctxt)

@@ -88,27 +88,54 @@ class AttrState(object):
         self.reg = reg
         self.ctype = ctype
         self.loaded = False
-        self.loader = None   # C expression producing the value
+        self.loader = None   # C expression producing the value of row u
         self.deps = ()       # index registers the loader reads
 
 
-class Pipeline(object):
-    """One generated kernel."""
+class RowLoop(object):
+    """Emits ``for (u = 0; u < U; ++u)`` around per-row code (nothing when a
+    pipeline processes one row per thread and iteration)."""
 
-    def __init__(self, program, name, source):
+    def __init__(self, pipe):
+        self.pipe = pipe
+
+    def __enter__(self):
+        if self.pipe.U > 1:
+            self.pipe.code.emit("#pragma unroll")
+            self.pipe.code.open("for (int u = 0; u < %d; ++u)" % self.pipe.U)
+        return self
+
+    def __exit__(self, *exc):
+        if self.pipe.U > 1:
+            self.pipe.code.close()
+
+
+class Pipeline(object):
+    """One generated kernel.
+
+    Each thread handles U rows per loop trip (U > 1 only for pipelines
+    without inner loops): row u of trip ``base`` is ``base + u*B + tid``, so
+    for every u a wavefront reads 64 consecutive rows.  Per-row registers are
+    arrays ``name[U]`` indexed by ``u``; all per-row code sits inside
+    ``rowLoop()`` blocks, so the U independent rows issue their loads
+    together.  When U == 1 the index is the constant ``u = 0``."""
+
+    def __init__(self, program, name, source, U=1):
         self.program = program
         self.name = name
         self.pipeId = len(program.pipes)
         self.bs = int(KernelCall.defaultBlockSize)
         if self.bs % WAVE or not (WAVE <= self.bs <= 1024):
             raise ValueError("block size must be a multiple of 64 in [64, 1024]")
+        self.U = max(1, int(U))
         self.defaultGrid = int(KernelCall.defaultGridSize or 0)
-        self.source = source                # dict(rows=host expr)
+        self.source = source
         self.params = OrderedDict()         # field -> (ctype, host expr)
-        self.regs = OrderedDict()           # name -> (ctype, init)
+        self.regs = OrderedDict()           # name -> (ctype, init, perRow)
         self.staticLds = []
         self.prologue = Code(1)
         self.code = Code(2)
+        self.iterationEnd = Code(2)
         self.epilogue = Code(1)
         self.attrs = {}
         self.indexRegs = OrderedDict()
@@ -116,12 +143,15 @@ class Pipeline(object):
         self.dynLdsExpr = None
         self.refillPoints = []
         self.annotations = []
-        self.hostPre = []                   # host statements before launch (grid/LDS helpers)
         self.innerDepth = 0
         self.param("n", "int64_t", source["rows"])
         self.tid = "tid_" + ident(source["tidName"])
         self.indexRegs[self.tid] = "int64_t"
-        self.regs[self.tid] = ("int64_t", None)
+        self.reg(self.tid, "int64_t")
+        self.A = "active[u]"
+
+    def rowLoop(self):
+        return RowLoop(self)
 
     # ---- parameters and registers
     def param(self, field, ctype, hostExpr):
@@ -130,10 +160,12 @@ class Pipeline(object):
         self.params[field] = (ctype, hostExpr)
         return "p." + field
 
-    def reg(self, name, ctype, init=None):
+    def reg(self, name, ctype, init=None, perRow=True):
+        """Declare a register (an array of U when perRow); returns the
+        expression for the current row (``name[u]``) or the scalar name."""
         if name not in self.regs:
-            self.regs[name] = (ctype, init)
-        return name
+            self.regs[name] = (ctype, init, perRow)
+        return name + "[u]" if self.regs[name][2] else name
 
     def emit(self, line):
         self.code.emit(line)
@@ -169,24 +201,27 @@ class Pipeline(object):
             raise KeyError("attribute %s (%d) is not available in pipeline %s" % (attr.name, attr.id, self.name))
         if not st.loaded:
             raise RuntimeError("attribute %s accessed before load" % attr.name)
-        return st.reg
+        return st.reg + "[u]"
 
     def need(self, attrs):
-        """Emit loads for attributes not yet in registers (active lanes only)."""
+        """Emit loads for attributes not yet in registers (active rows only)."""
         pending = []
+        seen = set()
         for a in attrs:
             st = self.attrs.get(a.id)
             if st is None:
                 raise KeyError("attribute %s (%d) is not available in pipeline %s" % (a.name, a.id, self.name))
-            if not st.loaded:
+            if not st.loaded and a.id not in seen:
                 pending.append(st)
+                seen.add(a.id)
         if not pending:
             return
-        self.code.open("if (active)")
-        for st in pending:
-            self.emit("%s = %s;" % (st.reg, st.loader))
-            st.loaded = True
-        self.code.close()
+        with self.rowLoop():
+            self.code.open("if (active[u])")
+            for st in pending:
+                self.emit("%s[u] = %s;" % (st.reg, st.loader))
+                st.loaded = True
+            self.code.close()
 
     def liveState(self, attrs):
         """Registers that carry the given attributes across a lane
@@ -205,6 +240,7 @@ class Pipeline(object):
 
     # ---- rendering
     def render(self):
+        U, B = self.U, self.bs
         k = Code()
         k.open("struct KP_%s" % self.name)
         for f, (ct, _) in self.params.items():
@@ -212,7 +248,7 @@ class Pipeline(object):
         k.close(";")
         if self.phased:
             k.emit("template <int PHASE>")
-        k.open("__global__ void __launch_bounds__(%d) %s(const KP_%s p)" % (self.bs, self.name, self.name))
+        k.open("__global__ void __launch_bounds__(%d) %s(const KP_%s p)" % (B, self.name, self.name))
         for d in self.staticLds:
             k.emit(d)
         if self.dynLdsExpr is not None:
@@ -222,15 +258,29 @@ class Pipeline(object):
         k.emit("(void)lane; (void)wave;")
         k.emit("dqc_plan_status* st = p.st;")
         k.emit("(void)st;")
-        for name, (ct, init) in self.regs.items():
-            k.emit("%s %s%s;" % (ct, name, "" if init is None else " = " + init))
+        if U == 1:
+            k.emit("constexpr int u = 0;")
+        for name, (ct, init, perRow) in self.regs.items():
+            if perRow:
+                k.emit("%s %s[%d];" % (ct, name, U))
+                if init is not None:
+                    k.emit("for (int i = 0; i < %d; ++i) %s[i] = %s;" % (U, name, init))
+            else:
+                k.emit("%s %s%s;" % (ct, name, "" if init is None else " = " + init))
         k.extend(self.prologue)
-        k.emit("const int64_t stride = (int64_t)gridDim.x * %d;" % self.bs)
-        k.open("for (int64_t base = (int64_t)blockIdx.x * %d;; base += stride)" % self.bs)
+        k.emit("const int64_t stride = (int64_t)gridDim.x * %d;" % (B * U))
+        k.open("for (int64_t base = (int64_t)blockIdx.x * %d;; base += stride)" % (B * U))
         k.emit("const bool more = base < p.n;")
-        k.emit("%s = base + (int64_t)threadIdx.x;" % self.tid)
-        k.emit("int active = more && %s < p.n;" % self.tid)
+        k.emit("int active[%d];" % U)
+        if U > 1:
+            k.emit("#pragma unroll")
+            k.open("for (int u = 0; u < %d; ++u)" % U)
+        k.emit("%s[u] = base + (int64_t)(u * %d) + (int64_t)threadIdx.x;" % (self.tid, B))
+        k.emit("active[u] = more && %s[u] < p.n;" % self.tid)
+        if U > 1:
+            k.close()
         k.extend(self.code)
+        k.extend(self.iterationEnd)
         k.emit("if (!more) break;")
         k.close()
         k.extend(self.epilogue)
@@ -291,10 +341,10 @@ class Program(object):
         self.hts.append(rec)
         return rec
 
-    def newPipeline(self, name, source):
+    def newPipeline(self, name, source, U=1):
         if len(self.pipes) >= 32:
             raise ValueError("too many pipelines in one plan")
-        pipe = Pipeline(self, name, source)
+        pipe = Pipeline(self, name, source, U)
         self.pipes.append(pipe)
         return pipe
 
@@ -541,7 +591,7 @@ def pipelineLaunch(prog, pipe, phase=None, rowsExpr=None, perCu=8):
         lines.append("kp.%s = (%s)(%s);" % (f, ct, expr))
     rows = rowsExpr or pipe.params["n"][1]
     lines.append("const unsigned g = gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, %d);"
-                 % (pipe.pipeId, pipe.pipeId, pipe.defaultGrid, rows, pipe.bs, perCu))
+                 % (pipe.pipeId, pipe.pipeId, pipe.defaultGrid, rows, pipe.bs * pipe.U, perCu))
     lds = "0" if pipe.dynLdsExpr is None else "(size_t)(%s)" % pipe.dynLdsExpr
     kname = pipe.name if phase is None else "%s<%d>" % (pipe.name, phase)
     lines.append("hipLaunchKernelGGL((%s), dim3(g), dim3(%d), %s, s, kp);" % (kname, pipe.bs, lds))
@@ -643,7 +693,46 @@ class HipCompiler(object):
 
 # ------------------------------------------------------------------ operators
 
+def pipelineShape(translator):
+    """Walk up from a pipeline's source translator to its sink and report
+    (hasInnerLoop, passFraction): whether an operator of the pipeline opens a
+    per-lane loop (multimatch probe, lane refill) and the product of the
+    configured selectivities of the selections before the first join."""
+    from dogqc_amd.hashJoins import EquiJoinTranslator
+    from dogqc_amd.hipDivergenceBuffer import LaneRefill
+    from dogqc_amd.relationalAlgebra import Join
+    loops, frac, beforeJoin = False, 1.0, True
+    t = translator
+    while t.parent is not None:
+        p = t.parent
+        if isinstance(p, LaneRefill):
+            loops = True
+        elif isinstance(p, EquiJoinTranslator):
+            if p.leftChild is t:
+                break
+            beforeJoin = False
+            if p.algExpr.multimatch and p.algExpr.joinType in (Join.INNER, Join.OUTER):
+                loops = True
+        elif isinstance(p, SelectionTranslator) and beforeJoin:
+            frac *= float(getattr(p.algExpr, "selectivity", 1.0))
+        elif isinstance(p, (AggregationTranslator, MaterializeTranslator)):
+            break
+        t = p
+    return loops, frac
+
+
+def rowsPerThread(translator):
+    loops, _ = pipelineShape(translator)
+    return 1 if loops else max(1, int(KernelCall.defaultRowsPerThread))
+
+
 class ScanTranslator(LiteralTranslator):
+    """Opens a pipeline over a base table.  Columns load lazily at first use;
+    when the configured selectivities say the filters before the first join
+    keep at least half of the rows, all columns load at once at the top
+    (lazy loading would then only add dependent memory round trips)."""
+
+    EAGER_PASS_FRACTION = 0.5
 
     def produce(self, ctxt):
         a = self.algExpr
@@ -651,23 +740,27 @@ class ScanTranslator(LiteralTranslator):
         tname = a.table["name"]
         ti = prog.table(tname)
         name = "krnl_%s%d" % (ident(tname), a.opId)
-        pipe = prog.newPipeline(name, {"rows": "L.rows[%d]" % ti, "tidName": "%s%d" % (tname, a.scanTableId)})
+        pipe = prog.newPipeline(name, {"rows": "L.rows[%d]" % ti, "tidName": "%s%d" % (tname, a.scanTableId)},
+                                rowsPerThread(self))
         ctxt.pipe = pipe
+        tid = pipe.tid + "[u]"
         for id, attr in a.outRelation.items():
             if attr.dataType == Type.STRING:
                 oi = prog.column(tname, attr.name, "offset", CType.UINT64)
                 ci = prog.column(tname, attr.name, "char", "char")
                 fo = pipe.param("c%d" % oi, "const uint64_t*", "C_%d" % oi)
                 fc = pipe.param("c%d" % ci, "const char*", "C_%d" % ci)
-                pipe.setLazy(attr, "dqc::string_scan(%s, %s, %s)" % (fo, fc, pipe.tid), [pipe.tid])
+                pipe.setLazy(attr, "dqc::string_scan(%s, %s, %s)" % (fo, fc, tid), [pipe.tid])
                 prog.charSource[attr.id] = ci
             else:
                 ct = columnType(attr.dataType)
                 ci = prog.column(tname, attr.name, None, ct)
                 f = pipe.param("c%d" % ci, "const %s*" % ct, "C_%d" % ci)
-                pipe.setLazy(attr, "(%s)%s[%s]" % (langType(attr.dataType, prog.decimalRepr), f, pipe.tid),
-                             [pipe.tid])
+                pipe.setLazy(attr, "(%s)%s[%s]" % (langType(attr.dataType, prog.decimalRepr), f, tid), [pipe.tid])
         pipe.param("st", "dqc_plan_status*", "st")
+        _, frac = pipelineShape(self)
+        if frac >= ScanTranslator.EAGER_PASS_FRACTION:
+            pipe.need(list(a.outRelation.values()))
         self.parent.consume(ctxt)
         closePipeline(ctxt, pipe)
 
@@ -726,7 +819,8 @@ class SelectionTranslator(UnaryTranslator):
         pipe.emit("// selection (opId %d)" % self.algExpr.opId)
         pipe.need(self.algExpr.conditionAttributes.values())
         cond = ctxt.exprEmitter().emit(self.algExpr.condition)
-        pipe.emit("if (active) active = (%s) ? 1 : 0;" % cond)
+        with pipe.rowLoop():
+            pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % cond)
         self.parent.consume(ctxt)
 
 
@@ -747,7 +841,8 @@ class MapTranslator(UnaryTranslator):
             prog.charSource[a.mapAttr.id] = prog.charSource[a.mapStringAttributes[0].id]
         st = pipe.declareAttr(a.mapAttr)
         expr = ctxt.exprEmitter().emit(a.expression)
-        pipe.emit("if (active) %s = (%s)(%s);" % (st.reg, st.ctype, expr))
+        with pipe.rowLoop():
+            pipe.emit("if (active[u]) %s[u] = (%s)(%s);" % (st.reg, st.ctype, expr))
         st.loaded = True
         self.parent.consume(ctxt)
 
@@ -770,10 +865,19 @@ class ProjectionTranslator(UnaryTranslator):
 
 
 class MaterializeTranslator(UnaryTranslator):
-    """Result materialisation: wave-aggregated append (one atomic per
-    wavefront, lane offsets from mbcnt) into result columns sized by the
-    estimate; the count keeps growing past the capacity so the host can
-    retry with the exact size (cudaTranslator.py:538-579, 599-610)."""
+    """Result materialisation (cudaTranslator.py:538-579, 599-610).
+
+    Rows are appended to a workgroup staging area in LDS (one LDS atomic per
+    wavefront and row slot u, positions from mbcnt); at the end of a loop
+    trip, when the stage is close to full or the input is exhausted, the
+    workgroup reserves one range of the result with a single global atomic
+    and copies the stage out with coalesced stores.  The reference spends one
+    global atomic per warp on a single counter, which serialises in L2.  A
+    wavefront that finds the stage full (possible after expanding joins)
+    appends straight to the result.  The count keeps growing past the result
+    capacity so the host can retry with the exact size."""
+
+    STAGE_BYTES = 16 * 1024
 
     def produce(self, ctxt):
         self.child.produce(ctxt)
@@ -791,36 +895,75 @@ class MaterializeTranslator(UnaryTranslator):
         prog.hostVars["rcap"] = "L.rcap = t->result_capacity > 0 ? t->result_capacity : std::max<int64_t>(%s, 1);" % est
         cols = []
         pipe.param("rcap", "uint64_t", "L.rcap")
+        rowBytes = 0
         for i, att in enumerate(attrs):
             st = pipe.attrs[att.id]
-            if att.dataType == Type.STRING:
-                ct = CType.STR_OFFS
-            else:
-                ct = st.ctype
+            ct = CType.STR_OFFS if att.dataType == Type.STRING else st.ctype
+            rowBytes += CType.size.get(ct, 8)
             bname = "res%d" % i
             prog.buffer(bname, ct, "L.rcap")
             f = pipe.param("r%d" % i, "%s*" % ct, "B_%s" % bname)
-            cols.append({"name": att.name, "type": att.dataType.name, "ctype": ct, "buffer": bname,
-                         "char_col": prog.charSource.get(att.id), "field": f, "reg": st.reg})
-        prog.result = {"columns": [{k: v for k, v in c.items() if k not in ("field", "reg")} for c in cols]}
-        pipe.emit("const uint64_t wm = dqc::ballot(active);")
-        pipe.code.open("if (wm)")
-        pipe.emit("const int wl = dqc::first_lane(wm);")
-        pipe.emit("uint64_t wb = 0;")
-        pipe.emit("if ((int)lane == wl) wb = atomicAdd((unsigned long long*)&st->result_count, "
-                  "(unsigned long long)dqc::popc64(wm));")
-        pipe.emit("wb = dqc::shfl(wb, wl);")
-        pipe.emit("const uint64_t wp = wb + dqc::lanes_below(wm);")
-        pipe.code.open("if (active && wp < p.rcap)")
-        for c in cols:
-            if c["ctype"] == CType.STR_OFFS:
-                ci = c["char_col"]
+            val = pipe.access(att)
+            if ct == CType.STR_OFFS:
+                ci = prog.charSource[att.id]
                 fc = pipe.param("c%d" % ci, "const char*", "C_%d" % ci)
-                pipe.emit("%s[wp] = dqc::to_string_offset(%s, %s);" % (c["field"], fc, c["reg"]))
-            else:
-                pipe.emit("%s[wp] = %s;" % (c["field"], c["reg"]))
-        pipe.code.close()
-        pipe.code.close()
+                val = "dqc::to_string_offset(%s, %s)" % (fc, val)
+            cols.append({"name": att.name, "type": att.dataType.name, "ctype": ct, "buffer": bname,
+                         "char_col": prog.charSource.get(att.id), "field": f, "val": val})
+        prog.result = {"columns": [{k: v for k, v in c.items() if k not in ("field", "val")} for c in cols]}
+        S = max(64, min(4096, self.STAGE_BYTES // max(1, rowBytes)))
+        S = max(S, 64)
+        for i, c in enumerate(cols):
+            pipe.staticLds.append("__shared__ %s stg%d[%d];" % (c["ctype"], i, S))
+        pipe.staticLds.append("__shared__ uint32_t stg_cnt;")
+        pipe.staticLds.append("__shared__ unsigned long long stg_base;")
+        pipe.prologue.emit("if (threadIdx.x == 0) stg_cnt = 0;")
+        pipe.prologue.emit("__syncthreads();")
+        with pipe.rowLoop():
+            pipe.emit("const uint64_t wm = dqc::ballot(active[u]);")
+            pipe.code.open("if (wm)")
+            pipe.emit("const int wl = dqc::first_lane(wm);")
+            pipe.emit("uint32_t so = 0;")
+            pipe.emit("if ((int)lane == wl) so = atomicAdd(&stg_cnt, dqc::popc64(wm));")
+            pipe.emit("const uint32_t sp = dqc::shfl(so, wl) + dqc::lanes_below(wm);")
+            pipe.code.open("if (active[u] && sp < %du)" % S)
+            for i, c in enumerate(cols):
+                pipe.emit("stg%d[sp] = %s;" % (i, c["val"]))
+            pipe.code.close()
+            pipe.emit("const uint64_t om = dqc::ballot(active[u] && sp >= %du);" % S)
+            pipe.code.open("if (om)")
+            pipe.emit("const int ol = dqc::first_lane(om);")
+            pipe.emit("unsigned long long ob = 0;")
+            pipe.emit("if ((int)lane == ol) ob = atomicAdd((unsigned long long*)&st->result_count, "
+                      "(unsigned long long)dqc::popc64(om));")
+            pipe.emit("const uint64_t wp = dqc::shfl(ob, ol) + dqc::lanes_below(om);")
+            pipe.code.open("if (active[u] && sp >= %du && wp < p.rcap)" % S)
+            for c in cols:
+                pipe.emit("%s[wp] = %s;" % (c["field"], c["val"]))
+            pipe.code.close()
+            pipe.code.close()
+            pipe.code.close()
+        # block-uniform flush at the end of every loop trip
+        ie = pipe.iterationEnd
+        ie.emit("__syncthreads();")
+        ie.open("")
+        ie.emit("const uint32_t sc = stg_cnt < %du ? stg_cnt : %du;" % (S, S))
+        ie.open("if (sc > %du || (!more && sc > 0u))" % max(0, S - pipe.bs * pipe.U))
+        ie.emit("if (threadIdx.x == 0) stg_base = atomicAdd((unsigned long long*)&st->result_count, "
+                "(unsigned long long)sc);")
+        ie.emit("__syncthreads();")
+        ie.open("for (uint32_t i = threadIdx.x; i < sc; i += %du)" % pipe.bs)
+        ie.emit("const uint64_t wp = stg_base + i;")
+        ie.open("if (wp < p.rcap)")
+        for i, c in enumerate(cols):
+            ie.emit("%s[wp] = stg%d[i];" % (c["field"], i))
+        ie.close()
+        ie.close()
+        ie.emit("__syncthreads();")
+        ie.emit("if (threadIdx.x == 0) stg_cnt = 0;")
+        ie.emit("__syncthreads();")
+        ie.close()
+        ie.close()
 
 
 class AggregationTranslator(UnaryTranslator):
@@ -891,12 +1034,14 @@ class AggregationTranslator(UnaryTranslator):
                                 _fillPattern(acc["ctype"], _identity(acc["kind"], acc["ctype"])))
                 self.accBuffers.append(b)
                 f = pipe.param("acc%d_%d" % (a.opId, acc["i"]), "%s*" % acc["ctype"], "B_" + b)
-                r = pipe.reg("racc%d_%d" % (a.opId, acc["i"]), acc["ctype"], _cIdentity(acc["kind"], acc["ctype"]))
+                r = pipe.reg("racc%d_%d" % (a.opId, acc["i"]), acc["ctype"], _cIdentity(acc["kind"], acc["ctype"]),
+                             perRow=False)
                 acc["field"], acc["reg"] = f, r
-            pipe.code.open("if (active)")
-            for acc, v in zip(self.accs, vals):
-                pipe.emit(_combine(acc["kind"], acc["reg"], v))
-            pipe.code.close()
+            with pipe.rowLoop():
+                pipe.code.open("if (active[u])")
+                for acc, v in zip(self.accs, vals):
+                    pipe.emit(_combine(acc["kind"], acc["reg"], v))
+                pipe.code.close()
             # end of kernel: wave reduction + one atomic per wavefront
             for acc in self.accs:
                 red = {"sum": "wave_sum", "count": "wave_sum", "min": "wave_min", "max": "wave_max"}[acc["kind"]]
@@ -928,7 +1073,9 @@ class AggregationTranslator(UnaryTranslator):
         self.keyBuffers = []
         if self.exact:
             tagExpr, self.unpack = packed
-            pipe.code.open("if (active)")
+            rl = pipe.rowLoop()
+            rl.__enter__()
+            pipe.code.open("if (active[u])")
             pipe.emit("const uint64_t gtag = %s;" % tagExpr(pipe))
             pipe.emit("const uint64_t gh = dqc::mix64(gtag);")
             pipe.emit("int ls = -1;")
@@ -948,6 +1095,7 @@ class AggregationTranslator(UnaryTranslator):
                       % (1 << idx))
             pipe.code.close()
             pipe.code.close()
+            rl.__exit__()
             self._ldsTier(ctxt, pipe, prog, ftags, fcap, est)
         else:
             keyFields = []
@@ -957,7 +1105,9 @@ class AggregationTranslator(UnaryTranslator):
                 self.keyBuffers.append((b, ct))
                 keyFields.append(pipe.param("aht%d_k%d" % (a.opId, k), "%s*" % ct, "B_" + b))
             self.keyFields = keyFields
-            pipe.code.open("if (active)")
+            rl = pipe.rowLoop()
+            rl.__enter__()
+            pipe.code.open("if (active[u])")
             pipe.emit("uint64_t gh = 0;")
             for g in groups:
                 pipe.emit("gh = dqc::hash_combine(gh, %s);" % _keyBits(g, pipe.access(g)))
@@ -973,6 +1123,7 @@ class AggregationTranslator(UnaryTranslator):
             pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
                       % (1 << idx))
             pipe.code.close()
+            rl.__exit__()
 
     def _ldsTier(self, ctxt, pipe, prog, ftags, fcap, est):
         """Workgroup LDS table: W slots (runtime, power of two), accumulators
@@ -1040,8 +1191,10 @@ class AggregationTranslator(UnaryTranslator):
         prog = ctxt.program
         idx = self.ht["index"]
         name = "krnl_aggregation%d" % a.opId
-        pipe = prog.newPipeline(name, {"rows": "L.htcap[%d]" % idx, "tidName": "agg%d" % a.opId})
+        pipe = prog.newPipeline(name, {"rows": "L.htcap[%d]" % idx, "tidName": "agg%d" % a.opId},
+                                rowsPerThread(self))
         ctxt.pipe = pipe
+        tid = pipe.tid + "[u]"
         pipe.param("st", "dqc_plan_status*", "st")
         accFields = []
         for acc, b in zip(self.accs, self.accBuffers):
@@ -1049,16 +1202,17 @@ class AggregationTranslator(UnaryTranslator):
         groups = list(a.groupAttributes.values())
         if a.doGroup:
             ftags = pipe.param("aht%d" % a.opId, "const uint64_t*", "B_aht%d_tags" % a.opId)
-            pipe.reg("gtag", CType.UINT64, "0")
-            pipe.emit("if (active) { gtag = %s[%s]; active = (gtag >> 63) ? 1 : 0; }" % (ftags, pipe.tid))
-            if self.exact:
-                for g, unpack in zip(groups, self.unpack):
-                    st = pipe.setComputed(g)
-                    pipe.emit("%s = %s;" % (st.reg, unpack("gtag")))
-            else:
+            gtag = pipe.reg("gtag", CType.UINT64, "0")
+            with pipe.rowLoop():
+                pipe.emit("if (active[u]) { %s = %s[%s]; active[u] = (%s >> 63) ? 1 : 0; }" % (gtag, ftags, tid, gtag))
+                if self.exact:
+                    for g, unpack in zip(groups, self.unpack):
+                        st = pipe.setComputed(g)
+                        pipe.emit("%s[u] = %s;" % (st.reg, unpack(gtag)))
+            if not self.exact:
                 for g, (b, ct), k in zip(groups, self.keyBuffers, range(len(groups))):
                     f = pipe.param("aht%d_k%d" % (a.opId, k), "const %s*" % ct, "B_" + b)
-                    pipe.setLazy(g, "%s[%s]" % (f, pipe.tid), [pipe.tid], ct)
+                    pipe.setLazy(g, "%s[%s]" % (f, tid), [pipe.tid], ct)
         countAcc = None
         for acc in self.accs:
             if acc["kind"] == "count":
@@ -1067,10 +1221,10 @@ class AggregationTranslator(UnaryTranslator):
             attr = a.aggregateAttributes[id]
             acc = self.accs[accIdx]
             ct = langType(attr.dataType, ctxt.decimalRepr)
-            loader = "(%s)%s[%s]" % (ct, accFields[accIdx], pipe.tid)
+            loader = "(%s)%s[%s]" % (ct, accFields[accIdx], tid)
             if id in a.avgAggregates:
-                loader = "(%s)(%s[%s] / (%s)%s[%s])" % (ct, accFields[accIdx], pipe.tid, ct,
-                                                        accFields[countAcc["i"]], pipe.tid)
+                loader = "(%s)(%s[%s] / (%s)%s[%s])" % (ct, accFields[accIdx], tid, ct,
+                                                        accFields[countAcc["i"]], tid)
             pipe.setLazy(attr, loader, [pipe.tid], ct)
         self.parent.consume(ctxt)
         closePipeline(ctxt, pipe)
