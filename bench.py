@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--rows-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--untuned", action="store_true", help="ignore the autotuner's cached configuration")
     return ap.parse_args()
 
 
@@ -87,10 +88,19 @@ def main():
     rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[args.workload].items()}
     g = workloads.generate(args.workload, rows)
     acc = MemoryAccessor(g)
-    comp = planlib.translate(args.workload)
-    mod = rt.PlanModule(comp.compile(args.workload))
+    from dogqc_amd import autotune
+    rec = None if args.untuned else autotune.loadCached(args.workload, acc)
+    if rec is not None:
+        comp = autotune.translateVariant(args.workload, rec["block"], rec["rows_per_thread"])
+        mod = rt.PlanModule(comp.compile(autotune.variantName(args.workload, rec["block"], rec["rows_per_thread"])))
+        tuning = autotune.tuningFromDict(rec["tuning"])
+        tuned = {"block": rec["block"], "rows_per_thread": rec["rows_per_thread"], "tuning": rec["tuning"]}
+    else:
+        comp = planlib.translate(args.workload)
+        mod = rt.PlanModule(comp.compile(args.workload))
+        tuning = rt.newTuning()
+        tuned = None
     inputs = rt.BoundInputs(mod, acc)
-    tuning = rt.newTuning()
 
     # first run sizes the tables (retries update `tuning`) and checks parity
     ex = rt.run(mod, inputs, tuning)
@@ -152,6 +162,7 @@ def main():
         "kernels_ms": kernel_ms,
         "graph_ms_per_step": graph_ms,
         "parity": parity,
+        "autotuned": tuned,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpuBaseline(args.workload, g, args.cpu_seconds)

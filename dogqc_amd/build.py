@@ -42,11 +42,37 @@ def buildPlans(names=None, jobs=None, force=False):
     return dict(zip(names, paths))
 
 
+def buildVariants(jobs=None):
+    """Compile-time variants of the tunable plans (autotune.VARIANTS)."""
+    from dogqc_amd import planlib, autotune
+    work = []
+    for name in planlib.TUNE_PLANS:
+        for block, rows in autotune.VARIANTS:
+            comp = autotune.translateVariant(name, block, rows)
+            work.append((comp, autotune.variantName(name, block, rows)))
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        return list(ex.map(lambda cn: cn[0].compile(cn[1]), work))
+
+
+def cleanStale(keep):
+    """Remove plan modules and sources no current plan refers to."""
+    keep = set(os.path.basename(p)[:-3] for p in keep)
+    if not os.path.isdir(plancache.PLANDIR):
+        return
+    for f in os.listdir(plancache.PLANDIR):
+        stem = f.rsplit(".", 1)[0]
+        if stem not in keep:
+            os.unlink(os.path.join(plancache.PLANDIR, f))
+
+
 def buildAll(verbose=True):
     t0 = time.time()
     buildRuntime()
     paths = buildPlans(force=True)
+    variants = buildVariants()
+    cleanStale(list(paths.values()) + variants)
     if verbose:
-        print("built libdqcrt.so and %d plan modules in %.1f s" % (len(paths), time.time() - t0),
+        print("built libdqcrt.so and %d plan modules in %.1f s" % (len(paths) + len(variants), time.time() - t0),
               file=sys.stderr)
     return paths

@@ -141,16 +141,20 @@ DQC_DEVICE_INLINE void raise_status(uint32_t* status, uint32_t bit) { atomicOr(s
 
 // Insert a tag; duplicates of an existing tag take the next free slot (the
 // unique-join build of the reference also stores duplicate keys,
-// hashing.cuh:81-100).  Returns the slot, or -1 when the table is full.
-DQC_DEVICE_INLINE int64_t ht_insert(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
-    uint32_t s = slot_of(h, cap);
-    for (uint32_t n = 0; n < cap; ++n) {
+// hashing.cuh:81-100).  Starts at slot s with n slots already tried.
+// Returns the slot, or -1 when the table is full.
+DQC_DEVICE_INLINE int64_t ht_insert_from(uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t s, uint32_t n) {
+    for (; n < cap; ++n) {
         uint64_t prev = atomicCAS((unsigned long long*)&tags[s], (unsigned long long)TAG_EMPTY,
                                   (unsigned long long)tag);
         if (prev == TAG_EMPTY) return (int64_t)s;
         s = next_slot(s, cap);
     }
     return -1;
+}
+
+DQC_DEVICE_INLINE int64_t ht_insert(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
+    return ht_insert_from(tags, cap, tag, slot_of(h, cap), 0u);
 }
 
 // Find the next slot holding `tag`, starting at *cursor (initialise the
@@ -172,10 +176,10 @@ DQC_DEVICE_INLINE int64_t ht_probe(const uint64_t* tags, uint32_t cap, uint64_t 
 }
 
 // Group slot in exact mode: the tag is the key, so find-or-claim is one CAS
-// per probed slot with no lock.  Returns -1 when the table is full.
-DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
-    uint32_t s = slot_of(h, cap);
-    for (uint32_t n = 0; n < cap; ++n) {
+// per probed slot with no lock.  Starts at slot s with n slots tried.
+// Returns -1 when the table is full.
+DQC_DEVICE_INLINE int64_t agg_slot_exact_from(uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t s, uint32_t n) {
+    for (; n < cap; ++n) {
         uint64_t t = __hip_atomic_load(&tags[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (t == tag) return (int64_t)s;
         if (t == TAG_EMPTY) {
@@ -186,6 +190,10 @@ DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t 
         s = next_slot(s, cap);
     }
     return -1;
+}
+
+DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
+    return agg_slot_exact_from(tags, cap, tag, slot_of(h, cap), 0u);
 }
 
 // Group slot in hashed mode.  The claiming lane CASes EMPTY->LOCKED, writes

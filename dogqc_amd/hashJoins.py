@@ -79,7 +79,10 @@ class EquiJoinTranslator(BinaryTranslator):
         self.keyReconstruct = [b for b in self.buildKeys if b.id not in stored and
                                any(b.id == x.id for x in above + condBuild)]
 
-    def _hashCode(self, pipe, keys, tagVar, hashVar):
+    def _hashCode(self, pipe, keys, tagVar, hashVar, tagIsReg=False):
+        """Emit the tag and hash of a key tuple.  With tagIsReg the tag goes
+        into the per-row register ``tagVar`` (for two-phase lookups)."""
+        tagDecl = tagVar if tagIsReg else "const uint64_t " + tagVar
         if self.exact:
             parts = ["dqc::TAG_TOP"]
             shift = 0
@@ -88,14 +91,14 @@ class EquiJoinTranslator(BinaryTranslator):
                 mask = "0xffffffffull" if w == 32 else "0xffull"
                 parts.append("(((uint64_t)(%s) & %s) << %d)" % (pipe.access(k), mask, shift))
                 shift += w
-            pipe.emit("const uint64_t %s = %s;" % (tagVar, " | ".join(parts)))
+            pipe.emit("%s = %s;" % (tagDecl, " | ".join(parts)))
             pipe.emit("const uint64_t %s = dqc::mix64(%s);" % (hashVar, tagVar))
         else:
             from dogqc_amd.hipTranslator import _keyBits
             pipe.emit("uint64_t %s = 0;" % hashVar)
             for k in keys:
                 pipe.emit("%s = dqc::hash_combine(%s, %s);" % (hashVar, hashVar, _keyBits(k, pipe.access(k))))
-            pipe.emit("const uint64_t %s = dqc::TAG_TOP | (%s >> 1);" % (tagVar, hashVar))
+            pipe.emit("%s = dqc::TAG_TOP | (%s >> 1);" % (tagDecl, hashVar))
 
     # ----------------------------------------------------------- build
     def consumeBuild(self, ctxt):
@@ -142,33 +145,49 @@ class EquiJoinTranslator(BinaryTranslator):
             b = prog.buffer("jht%d_p%d" % (J, k), ct, count)
             self.storedBuffers.append((b, ct))
             fields.append(pipe.param("jht%d_p%d" % (J, k), "%s*" % ct, "B_" + b))
-        with pipe.rowLoop():
-            pipe.code.open("if (active[u])")
-            self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
-            if kind == "unique":
-                pipe.emit("const int64_t js = dqc::ht_insert(%s, %s, jtag%d, jh%d);" % (ftags, fcap, J, J))
+        if kind == "unique":
+            # first CAS of every row of the thread first, collisions after
+            jtag = pipe.reg("jtag%d" % J, "uint64_t")
+            jc = pipe.reg("jc%d" % J, "uint32_t")
+            jprev = pipe.reg("jprev%d" % J, "uint64_t")
+            with pipe.rowLoop():
+                pipe.code.open("if (active[u])")
+                self._hashCode(pipe, self.buildKeys, jtag, "jh%d" % J, tagIsReg=True)
+                pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jc, J, fcap))
+                pipe.emit("%s = atomicCAS((unsigned long long*)&%s[%s], 0ull, (unsigned long long)%s);"
+                          % (jprev, ftags, jc, jtag))
+                pipe.code.close()
+            with pipe.rowLoop():
+                pipe.code.open("if (active[u])")
+                pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : dqc::ht_insert_from(%s, %s, %s, "
+                          "dqc::next_slot(%s, %s), 1u);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
                 pipe.code.open("if (js >= 0)")
                 for f, att in zip(fields, self.stored):
                     pipe.emit("%s[js] = %s;" % (f, pipe.access(att)))
                 pipe.code.close()
                 pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
                           "atomicOr(&st->ht_overflow_mask, %du); }" % (1 << idx))
-            else:
-                pipe.phased = True
-                pipe.phasedHt = ht
-                pipe.emit("const uint32_t jb = dqc::slot_of(jh%d, %s);" % (J, fcap))
-                pipe.code.open("if constexpr (PHASE == 0)")
-                pipe.emit("atomicAdd(&%s[2u * jb + 1u], 1u);" % fdir)
                 pipe.code.close()
-                pipe.code.open("else")
-                pipe.emit("const uint32_t jpos = %s[2u * jb] + atomicAdd(&%s[2u * jb + 1u], 1u);" % (fdir, fdir))
-                pipe.code.open("if (jpos < %s)" % fecap)
-                for f, att in zip(fields, self.stored):
-                    pipe.emit("%s[jpos] = %s;" % (f, pipe.access(att)))
-                pipe.code.close()
-                pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); "
-                          "atomicOr(&st->mm_overflow_mask, %du); }" % (1 << idx))
-                pipe.code.close()
+            return
+        # multimap: count pass (PHASE 0) and insert pass (PHASE 1) of one kernel
+        pipe.phased = True
+        pipe.phasedHt = ht
+        with pipe.rowLoop():
+            pipe.code.open("if (active[u])")
+            self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
+            pipe.emit("const uint32_t jb = dqc::slot_of(jh%d, %s);" % (J, fcap))
+            pipe.code.open("if constexpr (PHASE == 0)")
+            pipe.emit("atomicAdd(&%s[2u * jb + 1u], 1u);" % fdir)
+            pipe.code.close()
+            pipe.code.open("else")
+            pipe.emit("const uint32_t jpos = %s[2u * jb] + atomicAdd(&%s[2u * jb + 1u], 1u);" % (fdir, fdir))
+            pipe.code.open("if (jpos < %s)" % fecap)
+            for f, att in zip(fields, self.stored):
+                pipe.emit("%s[jpos] = %s;" % (f, pipe.access(att)))
+            pipe.code.close()
+            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); "
+                      "atomicOr(&st->mm_overflow_mask, %du); }" % (1 << idx))
+            pipe.code.close()
             pipe.code.close()
 
     # ----------------------------------------------------------- probe
@@ -230,20 +249,39 @@ class EquiJoinTranslator(BinaryTranslator):
             slot = pipe.reg(slotName, "int64_t", "-1")
             pipe.indexRegs[slotName] = "int64_t"
             self._bindBuild(pipe, fields, slotName)
+            needCheck = (not self.exact) or a.conditions is not None
+            jtag = pipe.reg("jtag%d" % J, "uint64_t")
+            jc = pipe.reg("jc%d" % J, "uint32_t")
+            jt = pipe.reg("jt%d" % J, "uint64_t")
+            if not needCheck:
+                # first slot of every row of the thread first (independent loads)
+                with pipe.rowLoop():
+                    pipe.emit("%s = -1;" % slot)
+                    pipe.code.open("if (active[u])")
+                    self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True)
+                    pipe.emit("%s = dqc::probe_start(jh%d, %s);" % (jc, J, fcap))
+                    pipe.emit("%s = %s[%s];" % (jt, ftags, jc))
+                    pipe.code.close()
             with pipe.rowLoop():
-                pipe.emit("%s = -1;" % slot)
-                pipe.code.open("if (active[u])")
-                self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
-                pipe.emit("uint32_t jc = dqc::probe_start(jh%d, %s), jv = 0;" % (J, fcap))
-                needCheck = (not self.exact) or a.conditions is not None
                 if needCheck:
+                    pipe.emit("%s = -1;" % slot)
+                pipe.code.open("if (active[u])")
+                if needCheck:
+                    self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True)
+                    pipe.emit("uint32_t jcur = dqc::probe_start(jh%d, %s), jv = 0;" % (J, fcap))
                     pipe.code.open("for (;;)")
-                    pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
+                    pipe.emit("%s = dqc::ht_probe(%s, %s, %s, jcur, jv);" % (slot, ftags, fcap, jtag))
                     pipe.emit("if (%s < 0) break;" % slot)
                     pipe.emit("if (%s) break;" % self._matchCond(ctxt, pipe, fields, slotName))
                     pipe.code.close()
                 else:
-                    pipe.emit("%s = dqc::ht_probe(%s, %s, jtag%d, jc, jv);" % (slot, ftags, fcap, J))
+                    pipe.code.open("if (%s == %s)" % (jt, jtag))
+                    pipe.emit("%s = (int64_t)%s;" % (slot, jc))
+                    pipe.code.close()
+                    pipe.code.open("else if (%s != 0ull)" % jt)
+                    pipe.emit("uint32_t jcur = dqc::next_slot(%s, %s), jv = 1;" % (jc, fcap))
+                    pipe.emit("%s = dqc::ht_probe(%s, %s, %s, jcur, jv);" % (slot, ftags, fcap, jtag))
+                    pipe.code.close()
                 if a.joinType == Join.ANTI:
                     pipe.emit("active[u] = %s < 0;" % slot)
                 else:
@@ -254,6 +292,12 @@ class EquiJoinTranslator(BinaryTranslator):
                         p = self.probeKeys[self.buildKeys.index(b)]
                         st = pipe.setComputed(b)
                         pipe.emit("if (active[u]) %s[u] = %s;" % (st.reg, pipe.access(p)))
+            if a.joinType == Join.INNER and self.exact:
+                # a group-by on this join's key can accumulate at the join slot
+                pipe.probedJoins.append({"join": self, "slot": slotName, "htIndex": idx, "opId": J,
+                                         "loopOps": pipe.loopOps,
+                                         "buildKeyIds": [b.id for b in self.buildKeys],
+                                         "probeKeyIds": [p.id for p in self.probeKeys]})
             self.parent.consume(ctxt)
             return
 
@@ -288,6 +332,7 @@ class EquiJoinTranslator(BinaryTranslator):
             raise RuntimeError("multimatch probes need a one-row-per-thread pipeline")
         probeSide = self.rightChild.algExpr.outRelation
         probeLive = [att for id, att in a.outRelation.items() if id in probeSide]
+        pipe.loopOps += 1
         if EquiJoinTranslator.usePushDownJoin and ctxt.innerLoopCount == 0:
             self._probePushDown(ctxt, pipe, fdir, fcap, fields, posName, probeLive)
         else:

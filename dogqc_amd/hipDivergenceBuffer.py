@@ -40,6 +40,7 @@ class LaneRefill(UnaryTranslator):
             self.parent.consume(ctxt)
             return
         pipe.annotations.append("LR%d" % X)
+        pipe.loopOps += 1
         live = pipe.liveState(self.algExpr.outRelation.values())
         if pipe.tid not in live:
             live[pipe.tid] = "int64_t"

@@ -144,6 +144,8 @@ class Pipeline(object):
         self.refillPoints = []
         self.annotations = []
         self.innerDepth = 0
+        self.probedJoins = []
+        self.loopOps = 0            # operators so far that reassign tuples to lanes
         self.param("n", "int64_t", source["rows"])
         self.tid = "tid_" + ident(source["tidName"])
         self.indexRegs[self.tid] = "int64_t"
@@ -1006,6 +1008,25 @@ class AggregationTranslator(UnaryTranslator):
                 self.accs.append({"kind": kind, "in": inAttr, "ctype": ct, "i": len(self.accs)})
             self.aggAcc[id] = index[key]
 
+    def _fusable(self, pipe, groups):
+        """A group-by whose keys are exactly the keys of a unique join probed
+        earlier in this pipeline (exact tags, no lane reassignment since) can
+        accumulate at that join's slot: equal keys always resolve to the same
+        slot, so slots are groups.  Small configured group counts keep the
+        LDS tier instead."""
+        a = self.algExpr
+        if getattr(a, "numgroupsFromCfg", False) and a.numgroups <= 4096:
+            return None
+        if pipe.innerDepth:
+            return None
+        gids = sorted(g.id for g in groups)
+        for pj in reversed(pipe.probedJoins):
+            if pj["loopOps"] != pipe.loopOps:
+                continue
+            if gids == sorted(pj["buildKeyIds"]) or gids == sorted(pj["probeKeyIds"]):
+                return pj
+        return None
+
     def consume(self, ctxt):
         a = self.algExpr
         pipe = ctxt.pipe
@@ -1015,9 +1036,6 @@ class AggregationTranslator(UnaryTranslator):
         ins = [acc["in"] for acc in self.accs if acc["in"] is not None]
         groups = list(a.groupAttributes.values())
         pipe.need(groups + ins)
-        ht = prog.newHt("agg", a.opId, "aht%d" % a.opId)
-        self.ht = ht
-        idx = ht["index"]
         est = prog.estimate(a)
         self.doGroup = a.doGroup
         vals = []
@@ -1026,6 +1044,13 @@ class AggregationTranslator(UnaryTranslator):
                 vals.append("(%s)1" % acc["ctype"])
             else:
                 vals.append("(%s)%s" % (acc["ctype"], pipe.access(acc["in"])))
+        self.fused = self._fusable(pipe, groups) if a.doGroup else None
+        if self.fused is not None:
+            self._consumeFused(ctxt, pipe, prog, vals)
+            return
+        ht = prog.newHt("agg", a.opId, "aht%d" % a.opId)
+        self.ht = ht
+        idx = ht["index"]
         if not a.doGroup:
             prog.hostVars["htcap%d" % idx] = "L.htcap[%d] = 1; L.mment[%d] = 0;" % (idx, idx)
             self.accBuffers = []
@@ -1067,35 +1092,50 @@ class AggregationTranslator(UnaryTranslator):
                             _fillPattern(acc["ctype"], _identity(acc["kind"], acc["ctype"])))
             self.accBuffers.append(b)
             acc["field"] = pipe.param("acc%d_%d" % (a.opId, acc["i"]), "%s*" % acc["ctype"], "B_" + b)
+        overflow = ("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
+                    % (1 << idx))
 
         packed = _packKeys(groups)
         self.exact = packed is not None
         self.keyBuffers = []
         if self.exact:
             tagExpr, self.unpack = packed
-            rl = pipe.rowLoop()
-            rl.__enter__()
-            pipe.code.open("if (active[u])")
-            pipe.emit("const uint64_t gtag = %s;" % tagExpr(pipe))
-            pipe.emit("const uint64_t gh = dqc::mix64(gtag);")
-            pipe.emit("int ls = -1;")
-            pipe.emit("if (p.lw) ls = dqc::lds_slot_exact(ltags, p.lw, gtag, gh);")
-            pipe.code.open("if (ls >= 0)")
-            pipe.emit("const uint32_t li = (uint32_t)ls * p.lr + (lane & (p.lr - 1u));")
-            for acc, v in zip(self.accs, vals):
-                pipe.emit(_ldsOp(acc["kind"], "&lacc%d[li]" % acc["i"], v))
-            pipe.code.close()
-            pipe.code.open("else")
-            pipe.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, gtag, gh);" % (ftags, fcap))
-            pipe.code.open("if (gs >= 0)")
-            for acc, v in zip(self.accs, vals):
-                pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
-            pipe.code.close()
-            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
-                      % (1 << idx))
-            pipe.code.close()
-            pipe.code.close()
-            rl.__exit__()
+            G = a.opId
+            gtag = pipe.reg("gtag%d" % G, "uint64_t")
+            gs0 = pipe.reg("gs0_%d" % G, "uint32_t")
+            gt0 = pipe.reg("gt0_%d" % G, "uint64_t")
+            gneed = pipe.reg("gneed%d" % G, "int")
+            # pass 1: LDS table, or the first global slot (loads of all rows
+            # of the thread issued together)
+            with pipe.rowLoop():
+                pipe.emit("%s = 0;" % gneed)
+                pipe.code.open("if (active[u])")
+                pipe.emit("%s = %s;" % (gtag, tagExpr(pipe)))
+                pipe.emit("const uint64_t gh = dqc::mix64(%s);" % gtag)
+                pipe.emit("int ls = -1;")
+                pipe.emit("if (p.lw) ls = dqc::lds_slot_exact(ltags, p.lw, %s, gh);" % gtag)
+                pipe.code.open("if (ls >= 0)")
+                pipe.emit("const uint32_t li = (uint32_t)ls * p.lr + (lane & (p.lr - 1u));")
+                for acc, v in zip(self.accs, vals):
+                    pipe.emit(_ldsOp(acc["kind"], "&lacc%d[li]" % acc["i"], v))
+                pipe.code.close()
+                pipe.code.open("else")
+                pipe.emit("%s = 1;" % gneed)
+                pipe.emit("%s = dqc::slot_of(gh, %s);" % (gs0, fcap))
+                pipe.emit("%s = %s[%s];" % (gt0, ftags, gs0))
+                pipe.code.close()
+                pipe.code.close()
+            # pass 2: resolve collisions / claim slots, update the group
+            with pipe.rowLoop():
+                pipe.code.open("if (%s)" % gneed)
+                pipe.emit("const int64_t gs = %s == %s ? (int64_t)%s : dqc::agg_slot_exact_from(%s, %s, %s, %s, 0u);"
+                          % (gt0, gtag, gs0, ftags, fcap, gtag, gs0))
+                pipe.code.open("if (gs >= 0)")
+                for acc, v in zip(self.accs, vals):
+                    pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
+                pipe.code.close()
+                pipe.emit(overflow)
+                pipe.code.close()
             self._ldsTier(ctxt, pipe, prog, ftags, fcap, est)
         else:
             keyFields = []
@@ -1105,25 +1145,45 @@ class AggregationTranslator(UnaryTranslator):
                 self.keyBuffers.append((b, ct))
                 keyFields.append(pipe.param("aht%d_k%d" % (a.opId, k), "%s*" % ct, "B_" + b))
             self.keyFields = keyFields
-            rl = pipe.rowLoop()
-            rl.__enter__()
+            with pipe.rowLoop():
+                pipe.code.open("if (active[u])")
+                pipe.emit("uint64_t gh = 0;")
+                for g in groups:
+                    pipe.emit("gh = dqc::hash_combine(gh, %s);" % _keyBits(g, pipe.access(g)))
+                pipe.emit("const uint64_t gtag = dqc::TAG_TOP | (gh >> 1);")
+                writes = " ".join("%s[s] = %s;" % (f, pipe.access(g)) for f, g in zip(keyFields, groups))
+                eqs = " && ".join(_keyEq(g, "%s[s]" % f, pipe.access(g)) for f, g in zip(keyFields, groups))
+                pipe.emit("const int64_t gs = dqc::agg_slot_hashed(%s, %s, gtag, gh, [&](uint32_t s) { %s }, "
+                          "[&](uint32_t s) { return %s; });" % (ftags, fcap, writes, eqs))
+                pipe.code.open("if (gs >= 0)")
+                for acc, v in zip(self.accs, vals):
+                    pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
+                pipe.code.close()
+                pipe.emit(overflow)
+                pipe.code.close()
+
+    def _consumeFused(self, ctxt, pipe, prog, vals):
+        """Accumulate at the slot of the probed unique join (see _fusable);
+        a per-slot touched byte marks groups that received rows."""
+        a = self.algExpr
+        pj = self.fused
+        pipe.annotations.append("GJ%d" % pj["opId"])
+        cap = "L.htcap[%d]" % pj["htIndex"]
+        self.accBuffers = []
+        for acc in self.accs:
+            b = prog.buffer("agg%d_%d" % (a.opId, acc["i"]), acc["ctype"], cap,
+                            _fillPattern(acc["ctype"], _identity(acc["kind"], acc["ctype"])))
+            self.accBuffers.append(b)
+            acc["field"] = pipe.param("acc%d_%d" % (a.opId, acc["i"]), "%s*" % acc["ctype"], "B_" + b)
+        self.touched = prog.buffer("agg%d_hit" % a.opId, CType.CHAR, cap, _fillPattern(CType.CHAR, 0))
+        fhit = pipe.param("agg%d_hit" % a.opId, "int8_t*", "B_" + self.touched)
+        with pipe.rowLoop():
             pipe.code.open("if (active[u])")
-            pipe.emit("uint64_t gh = 0;")
-            for g in groups:
-                pipe.emit("gh = dqc::hash_combine(gh, %s);" % _keyBits(g, pipe.access(g)))
-            pipe.emit("const uint64_t gtag = dqc::TAG_TOP | (gh >> 1);")
-            writes = " ".join("%s[s] = %s;" % (f, pipe.access(g)) for f, g in zip(keyFields, groups))
-            eqs = " && ".join(_keyEq(g, "%s[s]" % f, pipe.access(g)) for f, g in zip(keyFields, groups))
-            pipe.emit("const int64_t gs = dqc::agg_slot_hashed(%s, %s, gtag, gh, [&](uint32_t s) { %s }, "
-                      "[&](uint32_t s) { return %s; });" % (ftags, fcap, writes, eqs))
-            pipe.code.open("if (gs >= 0)")
+            pipe.emit("const int64_t gs = %s[u];" % pj["slot"])
             for acc, v in zip(self.accs, vals):
                 pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
+            pipe.emit("%s[gs] = 1;" % fhit)
             pipe.code.close()
-            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
-                      % (1 << idx))
-            pipe.code.close()
-            rl.__exit__()
 
     def _ldsTier(self, ctxt, pipe, prog, ftags, fcap, est):
         """Workgroup LDS table: W slots (runtime, power of two), accumulators
@@ -1189,10 +1249,9 @@ class AggregationTranslator(UnaryTranslator):
         occupied slot, AVG = SUM / COUNT (cudaTranslator.py:497-525)."""
         a = self.algExpr
         prog = ctxt.program
-        idx = self.ht["index"]
         name = "krnl_aggregation%d" % a.opId
-        pipe = prog.newPipeline(name, {"rows": "L.htcap[%d]" % idx, "tidName": "agg%d" % a.opId},
-                                rowsPerThread(self))
+        rows = "L.htcap[%d]" % (self.fused["htIndex"] if self.fused else self.ht["index"])
+        pipe = prog.newPipeline(name, {"rows": rows, "tidName": "agg%d" % a.opId}, rowsPerThread(self))
         ctxt.pipe = pipe
         tid = pipe.tid + "[u]"
         pipe.param("st", "dqc_plan_status*", "st")
@@ -1200,7 +1259,26 @@ class AggregationTranslator(UnaryTranslator):
         for acc, b in zip(self.accs, self.accBuffers):
             accFields.append(pipe.param("acc%d_%d" % (a.opId, acc["i"]), "const %s*" % acc["ctype"], "B_" + b))
         groups = list(a.groupAttributes.values())
-        if a.doGroup:
+        if self.fused is not None:
+            pj = self.fused
+            J = pj["opId"]
+            join = pj["join"]
+            fhit = pipe.param("agg%d_hit" % a.opId, "const int8_t*", "B_" + self.touched)
+            ftags = pipe.param("jht%d" % J, "const uint64_t*", "B_jht%d_tags" % J)
+            gtag = pipe.reg("gtag", CType.UINT64, "0")
+            keyIds = pj["buildKeyIds"] if sorted(g.id for g in groups) == sorted(pj["buildKeyIds"]) \
+                else pj["probeKeyIds"]
+            with pipe.rowLoop():
+                pipe.emit("if (active[u]) active[u] = %s[%s] != 0;" % (fhit, tid))
+                pipe.emit("if (active[u]) %s = %s[%s];" % (gtag, ftags, tid))
+                for g in groups:
+                    k = keyIds.index(g.id)
+                    shift = sum(PACK_BITS[b.dataType] for b in join.buildKeys[:k])
+                    w = PACK_BITS[join.buildKeys[k].dataType]
+                    src = "uint32_t" if w == 32 else "uint8_t"
+                    st = pipe.setComputed(g)
+                    pipe.emit("%s[u] = (%s)(%s)(%s >> %d);" % (st.reg, st.ctype, src, gtag, shift))
+        elif a.doGroup:
             ftags = pipe.param("aht%d" % a.opId, "const uint64_t*", "B_aht%d_tags" % a.opId)
             gtag = pipe.reg("gtag", CType.UINT64, "0")
             with pipe.rowLoop():
@@ -1219,7 +1297,6 @@ class AggregationTranslator(UnaryTranslator):
                 countAcc = acc
         for id, accIdx in self.aggAcc.items():
             attr = a.aggregateAttributes[id]
-            acc = self.accs[accIdx]
             ct = langType(attr.dataType, ctxt.decimalRepr)
             loader = "(%s)%s[%s]" % (ct, accFields[accIdx], tid)
             if id in a.avgAggregates:

@@ -23,17 +23,21 @@ from dogqc_amd.kernel import KernelCall
 
 
 @contextlib.contextmanager
-def knobs(pushdown=False, block=None, threshold=None):
-    old = (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold)
+def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None):
+    old = (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
+           KernelCall.defaultRowsPerThread)
     EquiJoinTranslator.usePushDownJoin = pushdown
     if block is not None:
         KernelCall.defaultBlockSize = block
     if threshold is not None:
         LaneRefill.threshold = threshold
+    if rowsPerThread is not None:
+        KernelCall.defaultRowsPerThread = rowsPerThread
     try:
         yield
     finally:
-        EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold = old
+        (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
+         KernelCall.defaultRowsPerThread) = old
 
 
 # ------------------------------------------------------------- feature plans
@@ -132,6 +136,14 @@ def _featurePlan(name, alg):
                                              alg.selection(scal.EqualsExpr(_A("t_cat"), _C("b", Type.CHAR)),
                                                            alg.scan("t")))))
         return p, {4: {"multimatch": False}, 6: {"numgroups": 16}}
+    if name == "f_groupjoin":
+        # group-by on the key of a unique join: accumulates at the join slot
+        p = alg.aggregation(["u_id"], [(Reduction.SUM, "t_val", "s"), (Reduction.MIN, "t_val", "mn"),
+                                        (Reduction.AVG, "t_f", "af"), (Reduction.COUNT, "", "c")],
+                            alg.join(("u_id", "t_id"), alg.scan("u"),
+                                     alg.selection(scal.LargerExpr(_A("t_val"), _C("-50.0", Type.DOUBLE)),
+                                                   alg.scan("t"))))
+        return p, {4: {"multimatch": False}}
     if name == "f_join_multi":
         # multimatch join on grp (many duplicates both sides), result rows materialised
         p = alg.projection(["t_id", "u_id", "t_grp"],
@@ -152,7 +164,7 @@ def _featurePlan(name, alg):
 
 
 FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
-                 "f_join_multi", "f_semi", "f_anti_multi"]
+                 "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi"]
 
 # name -> (workload or "feature", options)
 PLANS = OrderedDict()
@@ -185,13 +197,19 @@ def emptyAccessor(name):
     return featureEmpty() if kind == "feature" else workloads.emptyAccessor(kind)
 
 
-def translate(name, acc=None, decimalRepr=CType.FP64):
+# plans the autotuner explores compile-time variants of (built ahead of time)
+TUNE_PLANS = ["c0", "c1", "c2_none", "c2_refill", "c2_pushdown", "c2_both"]
+
+
+def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=None):
     """Resolve and translate a named plan; returns the HipCompiler with
-    generated source (call .compile() to get the module)."""
+    generated source (call .compile() to get the module).  ``block`` and
+    ``rowsPerThread`` override the plan's launch-shape knobs."""
     acc = acc or emptyAccessor(name)
     alg, p, cfg, opts = build(name, acc)
     plan = alg.resolveAlgebraPlan(p, cfg)
-    with knobs(pushdown=opts.get("pushdown", False), block=opts.get("block"), threshold=opts.get("threshold")):
+    with knobs(pushdown=opts.get("pushdown", False), block=block or opts.get("block"),
+               threshold=opts.get("threshold"), rowsPerThread=rowsPerThread):
         comp = HipCompiler(alg, "gfx950", decimalRepr)
         comp.setBuffers(opts.get("buffers", []))
         tp = alg.translateToCompilerPlan(plan, comp)

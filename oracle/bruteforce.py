@@ -102,7 +102,8 @@ class BruteForce(object):
                         if np.dtype(ct).kind == "f":
                             s = sum((Fraction(float(v)) for v in vals), Fraction(0))
                             if red == Reduction.AVG:
-                                o[id] = float(s / len(vals)) if vals else float("nan")
+                                # C: the rounded sum divided by the count
+                                o[id] = float(s) / float(len(vals)) if vals else float("nan")
                             else:
                                 o[id] = float(s)
                         else:

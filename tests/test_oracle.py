@@ -1,0 +1,259 @@
+"""Pins the CPU oracle (oracle/planeval.py) before it is trusted as the
+checker of the HIP path:
+
+  * against a row-at-a-time brute-force evaluator (nested-loop joins,
+    dictionary group-by, exact Fraction sums) on tiny inputs of every plan;
+  * its scalar arithmetic against gcc compiling the same C expressions
+    (the reference's generated code is C, so C decides the semantics);
+  * against closed forms: per-group counts and integer sums by bincount, the
+    expanding join's COUNT and SUM(b*p) factorised per key;
+  * against the independent multithreaded C oracle at full workload size;
+  * a mistake planted in the oracle (floor instead of truncating division)
+    must fail the gcc check.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import dogqc_amd.scalarAlgebra as scal
+from dogqc_amd import planlib, workloads
+from dogqc_amd.dbio import MemoryAccessor
+from dogqc_amd.relationalAlgebra import RelationalAlgebra, Reduction
+from dogqc_amd.types import Type
+from oracle import planeval, bruteforce, coracle
+from tests.helpers import assertSameResult, toleranceColumns
+
+SMALL = {"c0": {"cfg0": 400}, "c1": {"dim": 120, "fact": 900}, "c2": {"build": 150, "probe": 3000}}
+
+
+def resolved(name, acc):
+    alg, p, cfg, _ = planlib.build(name, acc)
+    return alg.resolveAlgebraPlan(p, cfg)
+
+
+@pytest.mark.parametrize("name", [n for n in planlib.PLANS if planlib.PLANS[n][0] != "cal"])
+def test_planeval_matches_bruteforce(name):
+    kind = planlib.PLANS[name][0]
+    if kind == "feature":
+        acc = planlib.featureData(160, 40, seed=3)
+    else:
+        acc = MemoryAccessor(workloads.generate(kind, SMALL[kind], seed=5))
+    plan = resolved(name, acc)
+    want = bruteforce.evaluate(plan, acc)
+    plan = resolved(name, acc)
+    got = planeval.evaluate(plan, acc)
+    # brute force sums exactly (Fractions) and rounds once, like math.fsum
+    assertSameResult(got, want, toleranceColumns(plan), rtol=0, atol=0, label=name)
+
+
+# ---------------------------------------------------------------- C semantics
+
+def _scalarTable():
+    t = {"i": (Type.INT, np.array([7, -7, 2147483647, -9, 0], dtype=np.int32)),
+         "j": (Type.INT, np.array([2, 2, 1, 4, 3], dtype=np.int32)),
+         "d": (Type.DATE, np.array([19950315, 19981231, 19920101, 20000229, 19960704], dtype=np.uint32)),
+         "c": (Type.CHAR, np.array([82, 65, -1, 0, 127], dtype=np.int8)),
+         "f": (Type.FLOAT, np.array([1.1, -2.5, 3.25, 1e-3, 7.0], dtype=np.float32)),
+         "x": (Type.DOUBLE, np.array([0.1, 2.675, -1e10, 3.0, 1.0 / 3.0])),
+         "b": (Type.INT64, np.array([3, -5, 1 << 40, -(1 << 41), 9], dtype=np.int64))}
+    from collections import OrderedDict
+    return MemoryAccessor(OrderedDict([("s", OrderedDict(t.items()))]))
+
+
+A, K = scal.AttrExpr, scal.ConstExpr
+CASES = [
+    ("div_int", scal.DivExpr(A("i"), A("j"))),
+    ("add_wrap", scal.AddExpr(A("i"), K("1", Type.INT))),
+    ("mix_int_double", scal.MulExpr(A("i"), A("x"))),
+    ("float_ops", scal.MulExpr(A("f"), scal.SubExpr(K("1.1f", Type.FLOAT), A("f")))),
+    ("float_double", scal.AddExpr(A("f"), A("x"))),
+    ("date_year", scal.ExtractExpr(A("d"), scal.ExtractType.YEAR)),
+    ("date_month", scal.ExtractExpr(A("d"), scal.ExtractType.MONTH)),
+    ("date_day", scal.ExtractExpr(A("d"), scal.ExtractType.DAY)),
+    ("int64_mul", scal.MulExpr(A("b"), A("b"))),
+    ("int64_div", scal.DivExpr(A("b"), K("7", Type.INT64))),
+    ("date_cmp", scal.SmallerEqualExpr(A("d"), K("19960101", Type.DATE))),
+    ("char_eq", scal.EqualsExpr(A("c"), K("R", Type.CHAR))),
+    ("case", scal.CaseExpr([(scal.LargerExpr(A("x"), K("1.0", Type.DOUBLE)), A("i"))], A("f"))),
+    ("neg_div_double", scal.DivExpr(A("x"), A("i"))),
+]
+
+_CTYPE = {"int32": "int32_t", "uint32": "uint32_t", "int8": "int8_t", "int64": "int64_t", "float32": "float",
+          "float64": "double", "bool": "int32_t"}
+
+
+def _cExpr(e):
+    """C text of a scalar expression in the reference's literal spelling
+    (scalarAlgebra.py:142-154); attributes are register variables."""
+    if isinstance(e, scal.AttrExpr):
+        return e.identifier + "_r"
+    if isinstance(e, scal.ConstExpr):
+        return {Type.FLOAT: "((float)%s)" % e.token, Type.CHAR: "'%s'" % e.token}.get(e.type, "(%s)" % e.token)
+    ops = {scal.MulExpr: "*", scal.AddExpr: "+", scal.SubExpr: "-", scal.DivExpr: "/", scal.SmallerEqualExpr: "<=",
+           scal.LargerExpr: ">", scal.EqualsExpr: "=="}
+    if type(e) in ops:
+        return "(%s %s %s)" % (_cExpr(e.leftChild), ops[type(e)], _cExpr(e.rightChild))
+    if isinstance(e, scal.ExtractExpr):
+        c = _cExpr(e.child)
+        return {scal.ExtractType.YEAR: "(%s / 10000)" % c, scal.ExtractType.MONTH: "((%s / 100) %% 100)" % c,
+                scal.ExtractType.DAY: "(%s %% 100)" % c}[e.extract]
+    if isinstance(e, scal.CaseExpr):
+        (w, t), = e.exprListWhenThen
+        return "(%s ? (double)%s : (double)%s)" % (_cExpr(w), _cExpr(t), _cExpr(e.exprElse))
+    raise NotImplementedError(type(e).__name__)
+
+
+def _gccValues(acc, cases):
+    """Compile the expressions with gcc (-ffp-contract=off, -fwrapv) and read
+    back their values as bit patterns, per row."""
+    s = acc.schema["s"]
+    n = s["size"]
+    lines = ["#include <stdio.h>", "#include <stdint.h>", "#include <string.h>", "int main(void) {"]
+    for a, typ in s["attributes"].items():
+        d = acc.column("s", a)
+        ct = {Type.INT: "int32_t", Type.DATE: "uint32_t", Type.CHAR: "int8_t", Type.FLOAT: "float",
+              Type.DOUBLE: "double", Type.INT64: "int64_t"}[typ]
+        reg = "double" if typ in (Type.FLOAT, Type.DOUBLE) else ct
+        vals = ", ".join(repr(float(v)) if typ in (Type.FLOAT, Type.DOUBLE) else str(int(v)) for v in d)
+        if typ == Type.FLOAT:
+            vals = ", ".join("%sf" % float.hex(float(v)) for v in d)
+        if typ == Type.DOUBLE:
+            vals = ", ".join(float.hex(float(v)) for v in d)
+        if typ == Type.INT64:
+            vals = ", ".join("%dLL" % int(v) for v in d)
+        lines.append("  static const %s %s_col[%d] = {%s};" % (ct, a, n, vals))
+        lines.append("  %s %s_r;" % (reg, a))
+    lines.append("  for (int r = 0; r < %d; ++r) {" % n)
+    for a in s["attributes"]:
+        lines.append("    %s_r = %s_col[r];" % (a, a))
+    for name, e, ct in cases:
+        lines.append("    { %s v = (%s)%s; unsigned long long bits = 0; memcpy(&bits, &v, sizeof(v));"
+                     " printf(\"%s %%d %%llu\\n\", r, bits); }" % (_CTYPE[ct], _CTYPE[ct], _cExpr(e), name))
+    lines.append("  }")
+    lines.append("  return 0;")
+    lines.append("}")
+    with tempfile.TemporaryDirectory() as td:
+        src = os.path.join(td, "x.c")
+        exe = os.path.join(td, "x")
+        with open(src, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        subprocess.run(["gcc", "-O0", "-std=c11", "-ffp-contract=off", "-fwrapv", "-o", exe, src], check=True)
+        out = subprocess.run([exe], check=True, stdout=subprocess.PIPE, text=True).stdout
+    vals = {}
+    for line in out.splitlines():
+        name, r, bits = line.split()
+        vals[(name, int(r))] = int(bits)
+    return vals
+
+
+def _oracleValues(acc, cases, evaluator=None):
+    alg = RelationalAlgebra(acc)
+    scan = alg.scan("s")
+    node = scan
+    for name, e in cases:
+        node = alg.map(name, e, node)
+    plan = alg.resolveAlgebraPlan(node, {})
+    rel, n = (evaluator or planeval.Evaluator(acc)).node(plan[-1])
+    out, types = {}, {}
+    m = plan[-1].child
+    while hasattr(m, "mapAttr"):
+        arr, ct = (evaluator or planeval.Evaluator(acc)).expr(m.expression, rel, n)
+        types[m.mapAttr.name] = ct
+        arr = np.broadcast_to(np.asarray(arr), (n,))
+        for r in range(n):
+            v = np.asarray(arr[r]).astype(ct if ct != "bool" else "int32")
+            out[(m.mapAttr.name, r)] = int.from_bytes(v.tobytes(), "little")
+        m = m.child
+    return out, types
+
+
+def test_scalar_semantics_match_gcc():
+    acc = _scalarTable()
+    got, types = _oracleValues(acc, CASES)
+    want = _gccValues(acc, [(n, e, types[n]) for n, e in CASES])
+    assert got == want
+
+
+def test_planted_division_mistake_is_caught(monkeypatch):
+    """An oracle that floors integer division (Python's //) must disagree
+    with C's truncation on negative operands."""
+    acc = _scalarTable()
+    cases = [("div_int", scal.DivExpr(A("i"), A("j")))]
+
+    class Floor(planeval.Evaluator):
+        def expr(self, e, rel, n):
+            if isinstance(e, scal.DivExpr):
+                a, at = self.expr(e.leftChild, rel, n)
+                b, bt = self.expr(e.rightChild, rel, n)
+                return (np.asarray(a).astype(np.int64) // np.asarray(b).astype(np.int64)).astype(np.int32), "int32"
+            return planeval.Evaluator.expr(self, e, rel, n)
+
+    got, types = _oracleValues(acc, cases, Floor(acc))
+    want = _gccValues(acc, [(n, e, types[n]) for n, e in cases])
+    assert got != want
+
+
+# --------------------------------------------------------------- closed forms
+
+def test_c0_counts_and_integer_sums_closed_form():
+    g = workloads.generate("c0", {"cfg0": 50000}, seed=8)
+    acc = MemoryAccessor(g)
+    res = planeval.evaluate(resolved("c0", acc), acc)
+    cols = {k: v[1] for k, v in g["cfg0"].items()}
+    keep = cols["date"] <= 19981110
+    key = cols["g1"][keep].astype(np.int64) * 256 + cols["g2"][keep].astype(np.int64)
+    cnt = np.bincount(key, minlength=3 * 256)
+    qty = np.bincount(key, weights=cols["qty"][keep].astype(np.float64), minlength=3 * 256)
+    for i in range(len(res["g1"])):
+        k = int(res["g1"][i]) * 256 + int(res["g2"][i])
+        assert res["count_order"][i] == cnt[k]
+        assert res["sum_qty"][i] == qty[k]          # integers below 2^53: exact in any order
+        assert res["avg_qty"][i] == qty[k] / cnt[k]
+    assert len(res["g1"]) == int((cnt > 0).sum()) == 6
+
+
+def test_c2_closed_form():
+    g = workloads.generate("c2", {"build": 3000, "probe": 60000}, seed=9)
+    acc = MemoryAccessor(g)
+    res = planeval.evaluate(resolved("c2_none", acc), acc)
+    from tests.test_gpu_parity import c2ClosedForm
+    cnt, s = c2ClosedForm(acc)
+    assert int(res["cnt"][0]) == cnt and int(res["sum_prod"][0]) == s
+
+
+def test_empty_no_group_aggregate_yields_one_row():
+    g = workloads.generate("c2", {"build": 10, "probe": 0}, seed=1)
+    acc = MemoryAccessor(g)
+    res = planeval.evaluate(resolved("c2_none", acc), acc)
+    assert int(res["cnt"][0]) == 0 and int(res["sum_prod"][0]) == 0
+
+
+# ------------------------------------------------------- C oracle agreement
+
+def test_c_oracle_agrees_at_full_size():
+    g = workloads.generate("c1")
+    acc = MemoryAccessor(g)
+    want = planeval.evaluate(resolved("c1", acc), acc)
+    keys, sums = coracle.c1({k: v[1] for k, v in g["dim"].items()}, {k: v[1] for k, v in g["fact"].items()})
+    o = np.argsort(keys)
+    w = np.argsort(want["d_key"])
+    assert np.array_equal(keys[o], want["d_key"][w])
+    assert np.allclose(sums[o], want["revenue"][w], rtol=1e-12, atol=0)
+
+    g2 = workloads.generate("c2", {"build": 200000, "probe": 2000000})
+    cnt, s = coracle.c2({k: v[1] for k, v in g2["build"].items()}, {k: v[1] for k, v in g2["probe"].items()})
+    from tests.test_gpu_parity import c2ClosedForm
+    assert (cnt, s) == c2ClosedForm(MemoryAccessor(g2))
+
+    g0 = workloads.generate("c0", {"cfg0": 300000})
+    acc0 = MemoryAccessor(g0)
+    res = planeval.evaluate(resolved("c0", acc0), acc0)
+    ref = coracle.c0({k: v[1] for k, v in g0["cfg0"].items()})
+    for i in range(len(res["g1"])):
+        r = ref[(int(res["g1"][i]), int(res["g2"][i]))]
+        assert int(res["count_order"][i]) == r[5]
+        np.testing.assert_allclose([res["sum_qty"][i], res["sum_base_price"][i], res["sum_disc_price"][i],
+                                    res["sum_charge"][i]], r[:4], rtol=1e-12)
