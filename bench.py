@@ -34,6 +34,12 @@ WORKLOADS = {
     "cal": {"table": "k", "kernel": "krnl_k1", "bytes_per_row": 8 + 8,
             "desc": "copy of one int64 column (8 B read + 8 B written per row): counter calibration"},
 }
+for _v, _d in (("none", "per-lane match loops"), ("refill", "lane refill after the filter"),
+               ("pushdown", "push-down match expansion"), ("both", "lane refill + push-down")):
+    WORKLOADS["c2_" + _v] = {
+        "table": "probe", "kernel": "krnl_probe2", "bytes_per_row": 4 + 4 + 8,
+        "desc": "divergent expanding join (%s): build 1M Zipf(1.0) keys over 100k values, probe 16M, "
+                "filter p_f < 10, COUNT + SUM(int64 b_pay * p_pay)" % _d}
 
 
 def parse():
@@ -85,8 +91,9 @@ def main():
     from dogqc_amd.dbio import MemoryAccessor
 
     spec = WORKLOADS[args.workload]
-    rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[args.workload].items()}
-    g = workloads.generate(args.workload, rows)
+    kind = planlib.PLANS[args.workload][0]
+    rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[kind].items()}
+    g = workloads.generate(kind, rows)
     acc = MemoryAccessor(g)
     from dogqc_amd import autotune
     rec = None if args.untuned else autotune.loadCached(args.workload, acc)
@@ -107,7 +114,7 @@ def main():
     got = runner.resultColumns(mod, ex, acc)
     ex.release()
     parity = checkParity(args.workload, g, got)
-    if args.workload == "cal":
+    if args.workload not in ("c0", "c1"):
         args.no_cpu = True
 
     stream = rt.Stream()
@@ -154,7 +161,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded PCG64 generator, dogqc_amd/workloads.py)",
-        "config": {"workload": spec["desc"], "rows": rows, "seed": workloads.SEEDS[args.workload],
+        "config": {"workload": spec["desc"], "rows": rows, "seed": workloads.SEEDS[kind],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
@@ -193,6 +200,10 @@ def checkParity(workload, g, got):
     import numpy as np
     from oracle import coracle
     cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
+    if workload.startswith("c2"):
+        cnt, s = coracle.c2(cols["build"], cols["probe"])
+        ok = int(got["cnt"][0]) == cnt and int(got["sum_prod"][0]) == s
+        return {"oracle": "C", "matches": int(cnt), "ok": bool(ok)}
     if workload == "cal":
         ok = np.array_equal(np.sort(got["k_v"]), cols["k"]["k_v"])
         return {"oracle": "identity", "rows": int(len(got["k_v"])), "ok": bool(ok)}

@@ -245,6 +245,27 @@ DQC_DEVICE_INLINE int lds_slot_exact(uint64_t* ltags, uint32_t cap, uint64_t tag
     return -1;
 }
 
+// Claim or find a workgroup LDS slot for a 32-bit key (0xffffffff = empty),
+// probing at most `tries` slots of a power-of-two table; -1 if none is free.
+constexpr uint32_t LKEY_EMPTY = 0xffffffffu;
+
+DQC_DEVICE_INLINE int lds_claim32(uint32_t* ltab, uint32_t mask, uint32_t key, int tries) {
+    uint32_t s = (key * 0x9e3779b1u) & mask;
+    for (int n = 0; n < tries; ++n) {
+        uint32_t t = __hip_atomic_load(&ltab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (t == key) return (int)s;
+        if (t == LKEY_EMPTY) {
+            uint32_t expected = LKEY_EMPTY;
+            if (__hip_atomic_compare_exchange_strong(&ltab[s], &expected, key, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ||
+                expected == key)
+                return (int)s;
+        }
+        s = (s + 1u) & mask;
+    }
+    return -1;
+}
+
 // ------------------------------------------------------------ atomics
 DQC_DEVICE_INLINE void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 DQC_DEVICE_INLINE void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }

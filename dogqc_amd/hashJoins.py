@@ -172,23 +172,94 @@ class EquiJoinTranslator(BinaryTranslator):
         # multimap: count pass (PHASE 0) and insert pass (PHASE 1) of one kernel
         pipe.phased = True
         pipe.phasedHt = ht
+        jb = pipe.reg("jb%d" % J, "uint32_t")
+        if pipe.loopOps == 0 and pipe.innerDepth == 0:
+            self._multiBuildStaged(pipe, J, idx, fdir, fcap, fecap, fields, jb)
+            return
         with pipe.rowLoop():
             pipe.code.open("if (active[u])")
             self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
-            pipe.emit("const uint32_t jb = dqc::slot_of(jh%d, %s);" % (J, fcap))
+            pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jb, J, fcap))
             pipe.code.open("if constexpr (PHASE == 0)")
-            pipe.emit("atomicAdd(&%s[2u * jb + 1u], 1u);" % fdir)
+            pipe.emit("atomicAdd(&%s[2u * %s + 1u], 1u);" % (fdir, jb))
             pipe.code.close()
             pipe.code.open("else")
-            pipe.emit("const uint32_t jpos = %s[2u * jb] + atomicAdd(&%s[2u * jb + 1u], 1u);" % (fdir, fdir))
-            pipe.code.open("if (jpos < %s)" % fecap)
-            for f, att in zip(fields, self.stored):
-                pipe.emit("%s[jpos] = %s;" % (f, pipe.access(att)))
-            pipe.code.close()
-            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); "
-                      "atomicOr(&st->mm_overflow_mask, %du); }" % (1 << idx))
+            pipe.emit("const uint32_t jpos = %s[2u * %s] + atomicAdd(&%s[2u * %s + 1u], 1u);" % (fdir, jb, fdir, jb))
+            self._storeEntry(pipe, idx, fecap, fields, "jpos")
             pipe.code.close()
             pipe.code.close()
+
+    def _storeEntry(self, pipe, idx, fecap, fields, pos):
+        pipe.code.open("if (%s < %s)" % (pos, fecap))
+        for f, att in zip(fields, self.stored):
+            pipe.emit("%s[%s] = %s;" % (f, pos, pipe.access(att)))
+        pipe.code.close()
+        pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); "
+                  "atomicOr(&st->mm_overflow_mask, %du); }" % (1 << idx))
+
+    def _multiBuildStaged(self, pipe, J, idx, fdir, fcap, fecap, fields, jb):
+        """Count and insert with per-trip workgroup aggregation: rows of one
+        loop trip that fall into the same bucket share one LDS counter, so a
+        bucket costs one global atomic per workgroup and trip (the insert pass
+        reserves a range and hands out ranks from LDS) instead of one per
+        row; skewed keys otherwise serialise on a single address in L2."""
+        W = 1
+        while W < pipe.bs * pipe.U:
+            W *= 2
+        pipe.staticLds.append("__shared__ uint32_t mmk%d[%d];" % (J, W))
+        pipe.staticLds.append("__shared__ uint32_t mmc%d[%d];" % (J, W))
+        pipe.staticLds.append("__shared__ uint32_t mmb%d[%d];" % (J, W))
+        pipe.prologue.emit("for (uint32_t i = threadIdx.x; i < %du; i += %du) { mmk%d[i] = dqc::LKEY_EMPTY; "
+                           "mmc%d[i] = 0u; }" % (W, pipe.bs, J, J))
+        pipe.prologue.emit("__syncthreads();")
+        ls = pipe.reg("mmls%d" % J, "int")
+        rank = pipe.reg("mmrank%d" % J, "uint32_t")
+        # pass A: bucket, LDS slot and rank within the trip (or a global atomic)
+        with pipe.rowLoop():
+            pipe.emit("%s = -1;" % ls)
+            pipe.code.open("if (active[u])")
+            self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
+            pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jb, J, fcap))
+            pipe.emit("%s = dqc::lds_claim32(mmk%d, %du, %s, 8);" % (ls, J, W - 1, jb))
+            pipe.code.open("if (%s >= 0)" % ls)
+            pipe.emit("%s = atomicAdd(&mmc%d[%s], 1u);" % (rank, J, ls))
+            pipe.code.close()
+            pipe.code.open("else")
+            pipe.code.open("if constexpr (PHASE == 0)")
+            pipe.emit("atomicAdd(&%s[2u * %s + 1u], 1u);" % (fdir, jb))
+            pipe.code.close()
+            pipe.code.open("else")
+            pipe.emit("const uint32_t jpos = %s[2u * %s] + atomicAdd(&%s[2u * %s + 1u], 1u);" % (fdir, jb, fdir, jb))
+            self._storeEntry(pipe, idx, fecap, fields, "jpos")
+            pipe.code.close()
+            pipe.code.close()
+            pipe.code.close()
+        # workgroup: one global atomic per distinct bucket of the trip
+        pipe.emit("__syncthreads();")
+        pipe.code.open("for (uint32_t i = threadIdx.x; i < %du; i += %du)" % (W, pipe.bs))
+        pipe.emit("const uint32_t k = mmk%d[i];" % J)
+        pipe.code.open("if (k != dqc::LKEY_EMPTY)")
+        pipe.code.open("if constexpr (PHASE == 0)")
+        pipe.emit("atomicAdd(&%s[2u * k + 1u], mmc%d[i]);" % (fdir, J))
+        pipe.code.close()
+        pipe.code.open("else")
+        pipe.emit("mmb%d[i] = %s[2u * k] + atomicAdd(&%s[2u * k + 1u], mmc%d[i]);" % (J, fdir, fdir, J))
+        pipe.code.close()
+        pipe.code.close()
+        pipe.code.close()
+        pipe.emit("__syncthreads();")
+        # pass B (insert): rows placed through LDS take their reserved position
+        with pipe.rowLoop():
+            pipe.code.open("if constexpr (PHASE == 1)")
+            pipe.code.open("if (active[u] && %s >= 0)" % ls)
+            pipe.emit("const uint32_t jpos = mmb%d[%s] + %s;" % (J, ls, rank))
+            self._storeEntry(pipe, idx, fecap, fields, "jpos")
+            pipe.code.close()
+            pipe.code.close()
+        pipe.emit("__syncthreads();")
+        pipe.emit("for (uint32_t i = threadIdx.x; i < %du; i += %du) { mmk%d[i] = dqc::LKEY_EMPTY; mmc%d[i] = 0u; }"
+                  % (W, pipe.bs, J, J))
+        pipe.emit("__syncthreads();")
 
     # ----------------------------------------------------------- probe
     def _storedFields(self, pipe):
