@@ -4,9 +4,11 @@ one MI355X, through the C ABI, with tables resident in HBM.
 A step is one full execution of the plan: the workspace init kernel, the dim
 build kernel, the fact probe/aggregate kernel and the group-table scan
 kernel, enqueued on one stream.  ``value`` is fact (probe-side) rows per
-second of the whole job.  Prints one JSON line.
+second of the whole job.  Prints one JSON line.  Other workloads (--workload)
+time their own plans the same way; the star join c3 times the plan captured
+into one hipGraph.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2_*|c3|cal]
                   [--rows-scale S] [--no-cpu]
 """
 import argparse
@@ -31,6 +33,10 @@ WORKLOADS = {
     "c0": {"table": "cfg0", "kernel": "krnl_cfg01", "bytes_per_row": 1 + 1 + 4 + 8 + 8 + 8 + 4,
            "desc": "scan-filter-groupby (Q1 shape): date <= cutoff, group by (g1, g2), "
                    "5 f64 sums + count"},
+    "c3": {"table": "sales", "kernel": "krnl_sales5", "bytes_per_row": 4 + 4 + 4 + 4 + 8, "graph": True,
+           "desc": "4-way star join + LIKE: sales 60M rows probe item 2M (LIKE '%green%' on 55-byte text, 5.3%), "
+                   "cust 200k, nation 25; star join materialised once, grouped by (n_key, c_year) and by i_key "
+                   "(SUM qty, SUM price, COUNT); whole plan replayed as one hipGraph"},
     "cal": {"table": "k", "kernel": "krnl_k1", "bytes_per_row": 8 + 8,
             "desc": "copy of one int64 column (8 B read + 8 B written per row): counter calibration"},
 }
@@ -59,11 +65,18 @@ def cpuBaseline(workload, g, seconds):
     """The C oracle (oracle/cpu_oracle.c) on all host threads, repeated on the
     full workload for about `seconds`; median time -> rows/s."""
     from oracle import coracle
+    from dogqc_amd import workloads
     threads = os.cpu_count() or 1
     cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
     if workload == "c1":
         run = lambda: coracle.c1(cols["dim"], cols["fact"], threads=threads)
         rows = len(cols["fact"]["f_fk"])
+    elif workload.startswith("c2"):
+        run = lambda: coracle.c2(cols["build"], cols["probe"], threads=threads)
+        rows = len(cols["probe"]["p_key"])
+    elif workload == "c3":
+        run = lambda: coracle.c3(cols, workloads.C3_WORD, threads=threads)
+        rows = len(cols["sales"]["s_item"])
     else:
         run = lambda: coracle.c0(cols["cfg0"], threads=threads)
         rows = len(cols["cfg0"]["qty"])
@@ -112,9 +125,10 @@ def main():
     # first run sizes the tables (retries update `tuning`) and checks parity
     ex = rt.run(mod, inputs, tuning)
     got = runner.resultColumns(mod, ex, acc)
+    temps = {t["name"]: runner.tempColumns(mod, ex, acc, t["name"]) for t in mod.desc["temps"]}
     ex.release()
-    parity = checkParity(args.workload, g, got)
-    if args.workload not in ("c0", "c1"):
+    parity = checkParity(args.workload, g, got, temps)
+    if args.workload == "cal":
         args.no_cpu = True
 
     stream = rt.Stream()
@@ -131,7 +145,7 @@ def main():
         rt.enqueue(mod, inputs, tuning, ws, stream, events[k])
     stream.sync()
     t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    eager_ms = (t1 - t0) * 1e3 / args.steps
     per = {name: [] for name in steps}
     for k in range(args.steps):
         for i, name in enumerate(steps):
@@ -140,6 +154,7 @@ def main():
 
     # the same plan captured once into a hipGraph and replayed
     graph_ms = graphReplay(mod, inputs, tuning, ws, stream, args.steps)
+    ms_per_step = graph_ms if spec.get("graph") else eager_ms
 
     probe_rows = rows[spec["table"]]
     dom = spec["kernel"]
@@ -168,6 +183,8 @@ def main():
                      "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
         "kernels_ms": kernel_ms,
         "graph_ms_per_step": graph_ms,
+        "eager_ms_per_step": eager_ms,
+        "timed": "hipGraph replay" if spec.get("graph") else "eager enqueue",
         "parity": parity,
         "autotuned": tuned,
     }
@@ -195,11 +212,36 @@ def graphReplay(mod, inputs, tuning, ws, stream, steps):
     return (t1 - t0) * 1e3 / steps
 
 
-def checkParity(workload, g, got):
+def sameGroups(got, want, keyCols, tolCols):
+    """Multiset equality of grouped results: keys and other exact columns
+    bit-exact, tolCols within 1e-9 relative."""
+    import numpy as np
+    if set(got) != set(want) or len(got[keyCols[0]]) != len(want[keyCols[0]]):
+        return False
+    og = np.lexsort([np.asarray(got[k]) for k in reversed(keyCols)])
+    ow = np.lexsort([np.asarray(want[k]) for k in reversed(keyCols)])
+    for k in want:
+        a, b = np.asarray(got[k])[og], np.asarray(want[k])[ow]
+        if k in tolCols:
+            if not np.allclose(a, b, rtol=1e-9, atol=0):
+                return False
+        elif not np.array_equal(a, b.astype(a.dtype)):
+            return False
+    return True
+
+
+def checkParity(workload, g, got, temps):
     """Bit-exact keys/counts, f64 sums within 1e-9 relative, against the C oracle."""
     import numpy as np
     from oracle import coracle
+    from dogqc_amd import workloads
     cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
+    if workload == "c3":
+        byItem, byNy = coracle.c3(cols, workloads.C3_WORD)
+        ok = sameGroups(got, byItem, ["i_key"], {"revenue"}) and \
+            sameGroups(temps["c3_by_nation_year"], byNy, ["n_key", "c_year"], {"revenue"})
+        return {"oracle": "C", "groups": [int(len(byItem["i_key"])), int(len(byNy["n_key"]))],
+                "joined_rows": int(byItem["cnt"].sum()), "ok": bool(ok)}
     if workload.startswith("c2"):
         cnt, s = coracle.c2(cols["build"], cols["probe"])
         ok = int(got["cnt"][0]) == cnt and int(got["sum_prod"][0]) == s

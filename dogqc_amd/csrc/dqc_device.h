@@ -245,6 +245,15 @@ DQC_DEVICE_INLINE int lds_slot_exact(uint64_t* ltags, uint32_t cap, uint64_t tag
     return -1;
 }
 
+// Orders this wavefront's earlier LDS writes before its later LDS accesses
+// by other lanes of the same wavefront (a wavefront's LDS operations complete
+// in issue order; the fences keep the compiler from reordering across).
+DQC_DEVICE_INLINE void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Claim or find a workgroup LDS slot for a 32-bit key (0xffffffff = empty),
 // probing at most `tries` slots of a power-of-two table; -1 if none is free.
 constexpr uint32_t LKEY_EMPTY = 0xffffffffu;

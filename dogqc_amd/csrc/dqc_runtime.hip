@@ -22,8 +22,9 @@
 
 #include "dogqc_amd.h"
 
-static_assert(sizeof(dqc_plan_tuning) == 1192, "dqc_plan_tuning layout");
-static_assert(sizeof(dqc_plan_status) == 280, "dqc_plan_status layout");
+static_assert(sizeof(dqc_plan_tuning) == 1256, "dqc_plan_tuning layout");
+static_assert(sizeof(dqc_plan_status) == 344, "dqc_plan_status layout");
+static_assert(sizeof(dqc_plan_layout) == 1648, "dqc_plan_layout layout");
 static_assert(sizeof(dqc_plan_inputs) == 24, "dqc_plan_inputs layout");
 
 namespace {
@@ -262,6 +263,12 @@ int dqc_query_run(void* module, const dqc_plan_inputs* in, dqc_plan_tuning* tun,
         if (st.result_count > (uint64_t)L.result_capacity && L.nresult > 0) {
             tun->result_capacity = (int64_t)st.result_count;
             retry = true;
+        }
+        for (int i = 0; i < L.ntemps && i < DQC_MAX_TEMPS; ++i) {
+            if (st.temp_count[i] > (uint64_t)L.temp_capacity[i]) {
+                tun->temp_capacity[i] = (int64_t)st.temp_count[i];
+                retry = true;
+            }
         }
         if (!retry) return DQC_OK;
         if (attempt >= max_retries)

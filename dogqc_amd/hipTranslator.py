@@ -147,6 +147,7 @@ class Pipeline(object):
         self.probedJoins = []
         self.loopOps = 0            # operators so far that reassign tuples to lanes
         self.param("n", "int64_t", source["rows"])
+        self.devRows = source.get("devRows")   # device-side row count (temp-table scans)
         self.tid = "tid_" + ident(source["tidName"])
         self.indexRegs[self.tid] = "int64_t"
         self.reg(self.tid, "int64_t")
@@ -260,6 +261,11 @@ class Pipeline(object):
         k.emit("(void)lane; (void)wave;")
         k.emit("dqc_plan_status* st = p.st;")
         k.emit("(void)st;")
+        if self.devRows is not None:
+            # rows appended by an earlier kernel of the plan, at most the capacity
+            k.emit("const int64_t nrows = std::min<int64_t>((int64_t)(%s), p.n);" % self.devRows)
+        else:
+            k.emit("const int64_t nrows = p.n;")
         if U == 1:
             k.emit("constexpr int u = 0;")
         for name, (ct, init, perRow) in self.regs.items():
@@ -272,13 +278,13 @@ class Pipeline(object):
         k.extend(self.prologue)
         k.emit("const int64_t stride = (int64_t)gridDim.x * %d;" % (B * U))
         k.open("for (int64_t base = (int64_t)blockIdx.x * %d;; base += stride)" % (B * U))
-        k.emit("const bool more = base < p.n;")
+        k.emit("const bool more = base < nrows;")
         k.emit("int active[%d];" % U)
         if U > 1:
             k.emit("#pragma unroll")
             k.open("for (int u = 0; u < %d; ++u)" % U)
         k.emit("%s[u] = base + (int64_t)(u * %d) + (int64_t)threadIdx.x;" % (self.tid, B))
-        k.emit("active[u] = more && %s[u] < p.n;" % self.tid)
+        k.emit("active[u] = more && %s[u] < nrows;" % self.tid)
         if U > 1:
             k.close()
         k.extend(self.code)
@@ -306,6 +312,7 @@ class Program(object):
         self.estimates = OrderedDict()   # opId -> C++ expression
         self.hostVars = OrderedDict()    # extra host layout statements
         self.result = None
+        self.temps = OrderedDict()       # temp table name -> dict(index, columns, est)
         self.charSource = {}             # attr id -> column slot index of its char heap
         self.profiles = []
         self.planNodes = []
@@ -328,6 +335,18 @@ class Program(object):
             self.strings[token] = "dqc_lit%d" % len(self.strings)
         name = self.strings[token]
         return "dqc::str_t{%s, %s + %d}" % (name, name, len(token.encode("latin-1")))
+
+    def newTemp(self, name, est):
+        if name in self.temps:
+            raise ValueError("temp table %s is created twice" % name)
+        if len(self.temps) >= 8:
+            raise ValueError("too many temp tables in one plan")
+        rec = {"index": len(self.temps), "name": name, "columns": [], "est": est}
+        self.temps[name] = rec
+        return rec
+
+    def tempColumns(self):
+        return [c for t in self.temps.values() for c in t["columns"]]
 
     # ---- workspace
     def buffer(self, name, ctype, count, init=None):
@@ -356,8 +375,13 @@ class Program(object):
             return "est_%d" % node.opId
         if isinstance(node, ra.Scan):
             if node.isTempScan:
-                raise NotImplementedError("temp table scans")
-            expr = "rows_%d" % self.table(node.table["name"])
+                tmp = self.temps.get(node.table["name"])
+                if tmp is None:
+                    raise ValueError("temp table %s is scanned before it is created" % node.table["name"])
+                i = tmp["index"]
+                expr = "(t->temp_capacity[%d] > 0 ? t->temp_capacity[%d] : %s)" % (i, i, tmp["est"])
+            else:
+                expr = "rows_%d" % self.table(node.table["name"])
         elif isinstance(node, ra.Selection):
             expr = "(int64_t)((double)%s * %r)" % (self.estimate(node.child), float(node.selectivity))
         elif isinstance(node, (ra.Map, ra.Projection, ra.Materialize)):
@@ -432,6 +456,7 @@ class Program(object):
         h.emit("    int64_t htcap[%d];" % max(1, len(self.hts)))
         h.emit("    int64_t mment[%d];" % max(1, len(self.hts)))
         h.emit("    int64_t rcap;")
+        h.emit("    int64_t tcap[%d];" % max(1, len(self.temps)))
         h.emit("    int64_t rows[%d];" % max(1, nt))
         h.emit("    uint32_t ldsw[%d];" % max(1, len(self.pipes)))
         h.emit("    uint32_t ldsr[%d];" % max(1, len(self.pipes)))
@@ -497,6 +522,12 @@ class Program(object):
         h.emit("out->nresult = %d;" % len(res))
         for i, c in enumerate(res):
             h.emit("out->result_offset[%d] = L.off[%d];" % (i, list(self.buffers).index(c["buffer"])))
+        h.emit("out->ntemps = %d;" % len(self.temps))
+        h.emit("out->ntempcols = %d;" % len(self.tempColumns()))
+        for t in self.temps.values():
+            h.emit("out->temp_capacity[%d] = L.tcap[%d];" % (t["index"], t["index"]))
+        for i, c in enumerate(self.tempColumns()):
+            h.emit("out->temp_offset[%d] = L.off[%d];" % (i, list(self.buffers).index(c["buffer"])))
         h.emit("out->nprofile = %d;" % len(self.profiles))
         if self.profiles:
             h.emit("out->profile_offset = L.off[%d];" % list(self.buffers).index("profile"))
@@ -538,6 +569,7 @@ class Program(object):
             "tables": list(self.tables),
             "columns": cols,
             "result": self.result["columns"] if self.result else [],
+            "temps": [{"name": t["name"], "index": t["index"], "columns": t["columns"]} for t in self.temps.values()],
             "steps": [s["name"] for s in self.steps],
             "pipelines": [{"name": p.name, "id": p.pipeId, "block": p.bs, "annotations": p.annotations}
                           for p in self.pipes],
@@ -740,8 +772,11 @@ class ScanTranslator(LiteralTranslator):
         a = self.algExpr
         prog = ctxt.program
         tname = a.table["name"]
-        ti = prog.table(tname)
         name = "krnl_%s%d" % (ident(tname), a.opId)
+        if a.isTempScan:
+            self._produceTemp(ctxt, name)
+            return
+        ti = prog.table(tname)
         pipe = prog.newPipeline(name, {"rows": "L.rows[%d]" % ti, "tidName": "%s%d" % (tname, a.scanTableId)},
                                 rowsPerThread(self))
         ctxt.pipe = pipe
@@ -760,11 +795,40 @@ class ScanTranslator(LiteralTranslator):
                 f = pipe.param("c%d" % ci, "const %s*" % ct, "C_%d" % ci)
                 pipe.setLazy(attr, "(%s)%s[%s]" % (langType(attr.dataType, prog.decimalRepr), f, tid), [pipe.tid])
         pipe.param("st", "dqc_plan_status*", "st")
+        self._finish(ctxt, pipe)
+
+    def _finish(self, ctxt, pipe):
         _, frac = pipelineShape(self)
         if frac >= ScanTranslator.EAGER_PASS_FRACTION:
-            pipe.need(list(a.outRelation.values()))
+            pipe.need(list(self.algExpr.outRelation.values()))
         self.parent.consume(ctxt)
         closePipeline(ctxt, pipe)
+
+    def _produceTemp(self, ctxt, name):
+        """Scan of a temp table written by an earlier pipeline of the plan
+        (the reference's tempscan over a materialised table,
+        cudaTranslator.py:168-182): the columns are workspace buffers and the
+        row count is read on the device from the temp table's append
+        counter, so the launch sequence needs no host round trip."""
+        a = self.algExpr
+        prog = ctxt.program
+        tmp = prog.temps.get(a.table["name"])
+        if tmp is None:
+            raise ValueError("temp table %s is scanned before it is created" % a.table["name"])
+        ti = tmp["index"]
+        pipe = prog.newPipeline(name, {"rows": "L.tcap[%d]" % ti, "devRows": "st->temp_count[%d]" % ti,
+                                       "tidName": "%s%d" % (a.table["name"], a.scanTableId)}, rowsPerThread(self))
+        ctxt.pipe = pipe
+        tid = pipe.tid + "[u]"
+        byName = {c["name"]: c for c in tmp["columns"]}
+        for id, attr in a.outRelation.items():
+            c = byName[attr.name]
+            f = pipe.param("t%d_%d" % (ti, c["pos"]), "const %s*" % c["ctype"], "B_%s" % c["buffer"])
+            pipe.setLazy(attr, "%s[%s]" % (f, tid), [pipe.tid], ctype=c["ctype"])
+            if c["char_col"] is not None:
+                prog.charSource[attr.id] = c["char_col"]
+        pipe.param("st", "dqc_plan_status*", "st")
+        self._finish(ctxt, pipe)
 
 
 def closePipeline(ctxt, pipe):
@@ -869,42 +933,58 @@ class ProjectionTranslator(UnaryTranslator):
 class MaterializeTranslator(UnaryTranslator):
     """Result materialisation (cudaTranslator.py:538-579, 599-610).
 
-    Rows are appended to a workgroup staging area in LDS (one LDS atomic per
-    wavefront and row slot u, positions from mbcnt); at the end of a loop
-    trip, when the stage is close to full or the input is exhausted, the
-    workgroup reserves one range of the result with a single global atomic
-    and copies the stage out with coalesced stores.  The reference spends one
-    global atomic per warp on a single counter, which serialises in L2.  A
-    wavefront that finds the stage full (possible after expanding joins)
-    appends straight to the result.  The count keeps growing past the result
-    capacity so the host can retry with the exact size."""
+    Every wavefront appends its rows to its own staging area in LDS
+    (positions from the ballot and mbcnt, the count kept wave-uniform in a
+    register); when the next ballot would not fit, and once at the end of
+    the kernel, the wavefront reserves one range of the output with a single
+    global atomic and copies the stage out with coalesced stores.  No
+    workgroup barrier is involved, so waves never wait for each other and
+    the append works inside per-lane match loops.  The reference spends one
+    global atomic per warp ballot on a single counter, which serialises in
+    L2.  The count keeps growing past the capacity so the host can retry
+    with the exact size."""
 
-    STAGE_BYTES = 16 * 1024
+    STAGE_BYTES_PER_WAVE = 4096
 
     def produce(self, ctxt):
         self.child.produce(ctxt)
 
     def consume(self, ctxt):
         a = self.algExpr
-        if a.matType != MaterializationType.RESULT:
-            raise NotImplementedError("temp tables are not supported by the HIP backend yet")
         pipe = ctxt.pipe
         prog = ctxt.program
-        pipe.emit("// materialize result (opId %d)" % a.opId)
         attrs = list(a.outRelation.values())
         pipe.need(attrs)
         est = prog.estimate(a)
-        prog.hostVars["rcap"] = "L.rcap = t->result_capacity > 0 ? t->result_capacity : std::max<int64_t>(%s, 1);" % est
+        isResult = a.matType == MaterializationType.RESULT
+        if isResult:
+            pipe.emit("// materialize result (opId %d)" % a.opId)
+            prog.hostVars["rcap"] = ("L.rcap = t->result_capacity > 0 ? t->result_capacity : "
+                                     "std::max<int64_t>(%s, 1);" % est)
+            cap = pipe.param("rcap", "uint64_t", "L.rcap")
+            counter = "st->result_count"
+            bprefix, fprefix = "res", "r"
+        else:
+            # temp table: same append path into workspace columns; string
+            # attributes keep their (start, end) pointers into the resident
+            # char heap
+            tmp = prog.newTemp(a.table["name"], est)
+            ti = tmp["index"]
+            pipe.emit("// materialize temp table %s (opId %d)" % (a.table["name"], a.opId))
+            prog.hostVars["tcap%d" % ti] = ("L.tcap[%d] = std::max<int64_t>(1, t->temp_capacity[%d] > 0 ? "
+                                           "t->temp_capacity[%d] : %s);" % (ti, ti, ti, est))
+            cap = pipe.param("tcap%d" % ti, "uint64_t", "L.tcap[%d]" % ti)
+            counter = "st->temp_count[%d]" % ti
+            bprefix, fprefix = "tmp%d_" % ti, "w%d_" % ti
         cols = []
-        pipe.param("rcap", "uint64_t", "L.rcap")
         rowBytes = 0
         for i, att in enumerate(attrs):
             st = pipe.attrs[att.id]
-            ct = CType.STR_OFFS if att.dataType == Type.STRING else st.ctype
-            rowBytes += CType.size.get(ct, 8)
-            bname = "res%d" % i
-            prog.buffer(bname, ct, "L.rcap")
-            f = pipe.param("r%d" % i, "%s*" % ct, "B_%s" % bname)
+            ct = CType.STR_OFFS if (isResult and att.dataType == Type.STRING) else st.ctype
+            rowBytes += CType.size.get(ct, 8) if ct != CType.STR_TYPE else 16
+            bname = "%s%d" % (bprefix, i)
+            prog.buffer(bname, ct, "L.rcap" if isResult else "L.tcap[%d]" % ti)
+            f = pipe.param("%s%d" % (fprefix, i), "%s*" % ct, "B_%s" % bname)
             val = pipe.access(att)
             if ct == CType.STR_OFFS:
                 ci = prog.charSource[att.id]
@@ -912,60 +992,56 @@ class MaterializeTranslator(UnaryTranslator):
                 val = "dqc::to_string_offset(%s, %s)" % (fc, val)
             cols.append({"name": att.name, "type": att.dataType.name, "ctype": ct, "buffer": bname,
                          "char_col": prog.charSource.get(att.id), "field": f, "val": val})
-        prog.result = {"columns": [{k: v for k, v in c.items() if k not in ("field", "val")} for c in cols]}
-        S = max(64, min(4096, self.STAGE_BYTES // max(1, rowBytes)))
-        S = max(S, 64)
+        public = [{k: v for k, v in c.items() if k not in ("field", "val")} for c in cols]
+        if isResult:
+            prog.result = {"columns": public}
+        else:
+            base = len(prog.tempColumns())
+            for i, c in enumerate(public):
+                c["pos"] = i
+                c["slot"] = base + i
+            tmp["columns"] = public
+        NW = pipe.bs // WAVE
+        SW = max(64, min(1024, (self.STAGE_BYTES_PER_WAVE // max(1, rowBytes)) // 64 * 64))
         for i, c in enumerate(cols):
-            pipe.staticLds.append("__shared__ %s stg%d[%d];" % (c["ctype"], i, S))
-        pipe.staticLds.append("__shared__ uint32_t stg_cnt;")
-        pipe.staticLds.append("__shared__ unsigned long long stg_base;")
-        pipe.prologue.emit("if (threadIdx.x == 0) stg_cnt = 0;")
-        pipe.prologue.emit("__syncthreads();")
+            pipe.staticLds.append("__shared__ %s stg%d[%d][%d];" % (c["ctype"], i, NW, SW))
+        wcnt = pipe.reg("stg_n", "uint32_t", "0u", perRow=False)
+
+        def flush(code):
+            # one global atomic per wavefront flush, coalesced copy-out by the
+            # lanes currently executing (all 64 unless inside divergent code)
+            code.emit("dqc::wave_lds_sync();")
+            code.emit("const uint64_t fx = dqc::ballot(true);")
+            code.emit("const int fl = dqc::first_lane(fx);")
+            code.emit("unsigned long long wb = 0;")
+            code.emit("if ((int)lane == fl) wb = atomicAdd((unsigned long long*)&%s, (unsigned long long)%s);"
+                      % (counter, wcnt))
+            code.emit("wb = dqc::uniform64(dqc::shfl(wb, fl));")
+            code.open("for (uint32_t i = dqc::lanes_below(fx); i < %s; i += dqc::popc64(fx))" % wcnt)
+            code.emit("const uint64_t wp = wb + i;")
+            code.open("if (wp < %s)" % cap)
+            for i, c in enumerate(cols):
+                code.emit("%s[wp] = stg%d[wave][i];" % (c["field"], i))
+            code.close()
+            code.close()
+            code.emit("dqc::wave_lds_sync();")
+            code.emit("%s = 0u;" % wcnt)
+
         with pipe.rowLoop():
             pipe.emit("const uint64_t wm = dqc::ballot(active[u]);")
-            pipe.code.open("if (wm)")
-            pipe.emit("const int wl = dqc::first_lane(wm);")
-            pipe.emit("uint32_t so = 0;")
-            pipe.emit("if ((int)lane == wl) so = atomicAdd(&stg_cnt, dqc::popc64(wm));")
-            pipe.emit("const uint32_t sp = dqc::shfl(so, wl) + dqc::lanes_below(wm);")
-            pipe.code.open("if (active[u] && sp < %du)" % S)
+            pipe.emit("const uint32_t wn = dqc::popc64(wm);")
+            pipe.code.open("if (%s + wn > %du)" % (wcnt, SW))
+            flush(pipe.code)
+            pipe.code.close()
+            pipe.code.open("if (active[u])")
+            pipe.emit("const uint32_t sp = %s + dqc::lanes_below(wm);" % wcnt)
             for i, c in enumerate(cols):
-                pipe.emit("stg%d[sp] = %s;" % (i, c["val"]))
+                pipe.emit("stg%d[wave][sp] = %s;" % (i, c["val"]))
             pipe.code.close()
-            pipe.emit("const uint64_t om = dqc::ballot(active[u] && sp >= %du);" % S)
-            pipe.code.open("if (om)")
-            pipe.emit("const int ol = dqc::first_lane(om);")
-            pipe.emit("unsigned long long ob = 0;")
-            pipe.emit("if ((int)lane == ol) ob = atomicAdd((unsigned long long*)&st->result_count, "
-                      "(unsigned long long)dqc::popc64(om));")
-            pipe.emit("const uint64_t wp = dqc::shfl(ob, ol) + dqc::lanes_below(om);")
-            pipe.code.open("if (active[u] && sp >= %du && wp < p.rcap)" % S)
-            for c in cols:
-                pipe.emit("%s[wp] = %s;" % (c["field"], c["val"]))
-            pipe.code.close()
-            pipe.code.close()
-            pipe.code.close()
-        # block-uniform flush at the end of every loop trip
-        ie = pipe.iterationEnd
-        ie.emit("__syncthreads();")
-        ie.open("")
-        ie.emit("const uint32_t sc = stg_cnt < %du ? stg_cnt : %du;" % (S, S))
-        ie.open("if (sc > %du || (!more && sc > 0u))" % max(0, S - pipe.bs * pipe.U))
-        ie.emit("if (threadIdx.x == 0) stg_base = atomicAdd((unsigned long long*)&st->result_count, "
-                "(unsigned long long)sc);")
-        ie.emit("__syncthreads();")
-        ie.open("for (uint32_t i = threadIdx.x; i < sc; i += %du)" % pipe.bs)
-        ie.emit("const uint64_t wp = stg_base + i;")
-        ie.open("if (wp < p.rcap)")
-        for i, c in enumerate(cols):
-            ie.emit("%s[wp] = stg%d[i];" % (c["field"], i))
-        ie.close()
-        ie.close()
-        ie.emit("__syncthreads();")
-        ie.emit("if (threadIdx.x == 0) stg_cnt = 0;")
-        ie.emit("__syncthreads();")
-        ie.close()
-        ie.close()
+            pipe.emit("%s += wn;" % wcnt)
+        pipe.epilogue.open("if (%s > 0u)" % wcnt)
+        flush(pipe.epilogue)
+        pipe.epilogue.close()
 
 
 class AggregationTranslator(UnaryTranslator):
@@ -975,7 +1051,9 @@ class AggregationTranslator(UnaryTranslator):
     counter; MIN/MAX their own.  Three tiers, chosen per row at run time:
       * no grouping: per-lane register accumulators, one wave reduction and
         one global atomic per wavefront at the end of the kernel;
-      * grouping with keys that pack into a 63-bit exact tag: a workgroup
+      * grouping with keys that pack into an exact tag (63 bits with the
+        top bit set, or exactly 64 bits stored as key + 1 with the single
+        key whose tag would be 0 kept in an extra slot at index cap): a workgroup
         LDS table of W tag slots whose accumulators are replicated R times
         (lane i uses replica i mod R, so lanes of one wavefront rarely hit the
         same LDS word), flushed to the global table once per workgroup; rows
@@ -1078,28 +1156,30 @@ class AggregationTranslator(UnaryTranslator):
             return
 
         # grouped
+        packed = _packKeys(groups)
+        self.exact = packed is not None
+        self.exact64 = self.exact and packed[2]
+        extra = " + 1" if self.exact64 else ""
         factor = "fac(t->ht_factor[%d])" % idx
         prog.hostVars["htcap%d" % idx] = (
             "L.htcap[%d] = t->ht_capacity[%d] > 0 ? t->ht_capacity[%d] : "
             "std::max<int64_t>(64, (int64_t)((double)%s * 2.0 * %s)); L.mment[%d] = 0;\n"
             "if (L.htcap[%d] >= (int64_t)0xffffffffll) return DQC_ENOMEM;" % (idx, idx, idx, est, factor, idx, idx))
-        tags = prog.buffer("aht%d_tags" % a.opId, CType.UINT64, "L.htcap[%d]" % idx, "0x0ull")
+        tags = prog.buffer("aht%d_tags" % a.opId, CType.UINT64, "L.htcap[%d]%s" % (idx, extra), "0x0ull")
         ftags = pipe.param("aht%d" % a.opId, "uint64_t*", "B_" + tags)
         fcap = pipe.param("aht%d_cap" % a.opId, "uint32_t", "L.htcap[%d]" % idx)
         self.accBuffers = []
         for acc in self.accs:
-            b = prog.buffer("agg%d_%d" % (a.opId, acc["i"]), acc["ctype"], "L.htcap[%d]" % idx,
+            b = prog.buffer("agg%d_%d" % (a.opId, acc["i"]), acc["ctype"], "L.htcap[%d]%s" % (idx, extra),
                             _fillPattern(acc["ctype"], _identity(acc["kind"], acc["ctype"])))
             self.accBuffers.append(b)
             acc["field"] = pipe.param("acc%d_%d" % (a.opId, acc["i"]), "%s*" % acc["ctype"], "B_" + b)
         overflow = ("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
                     % (1 << idx))
 
-        packed = _packKeys(groups)
-        self.exact = packed is not None
         self.keyBuffers = []
         if self.exact:
-            tagExpr, self.unpack = packed
+            tagExpr, self.unpack, _ = packed
             G = a.opId
             gtag = pipe.reg("gtag%d" % G, "uint64_t")
             gs0 = pipe.reg("gs0_%d" % G, "uint32_t")
@@ -1113,7 +1193,11 @@ class AggregationTranslator(UnaryTranslator):
                 pipe.emit("%s = %s;" % (gtag, tagExpr(pipe)))
                 pipe.emit("const uint64_t gh = dqc::mix64(%s);" % gtag)
                 pipe.emit("int ls = -1;")
-                pipe.emit("if (p.lw) ls = dqc::lds_slot_exact(ltags, p.lw, %s, gh);" % gtag)
+                if self.exact64:
+                    # tag 0 (key ~0) bypasses the LDS table and uses slot cap
+                    pipe.emit("if (p.lw && %s != 0) ls = dqc::lds_slot_exact(ltags, p.lw, %s, gh);" % (gtag, gtag))
+                else:
+                    pipe.emit("if (p.lw) ls = dqc::lds_slot_exact(ltags, p.lw, %s, gh);" % gtag)
                 pipe.code.open("if (ls >= 0)")
                 pipe.emit("const uint32_t li = (uint32_t)ls * p.lr + (lane & (p.lr - 1u));")
                 for acc, v in zip(self.accs, vals):
@@ -1121,8 +1205,12 @@ class AggregationTranslator(UnaryTranslator):
                 pipe.code.close()
                 pipe.code.open("else")
                 pipe.emit("%s = 1;" % gneed)
-                pipe.emit("%s = dqc::slot_of(gh, %s);" % (gs0, fcap))
-                pipe.emit("%s = %s[%s];" % (gt0, ftags, gs0))
+                if self.exact64:
+                    pipe.emit("%s = %s != 0 ? dqc::slot_of(gh, %s) : %s;" % (gs0, gtag, fcap, fcap))
+                    pipe.emit("%s = %s != 0 ? %s[%s] : 0ull;" % (gt0, gtag, ftags, gs0))
+                else:
+                    pipe.emit("%s = dqc::slot_of(gh, %s);" % (gs0, fcap))
+                    pipe.emit("%s = %s[%s];" % (gt0, ftags, gs0))
                 pipe.code.close()
                 pipe.code.close()
             # pass 2: resolve collisions / claim slots, update the group
@@ -1133,6 +1221,8 @@ class AggregationTranslator(UnaryTranslator):
                 pipe.code.open("if (gs >= 0)")
                 for acc, v in zip(self.accs, vals):
                     pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
+                if self.exact64:
+                    pipe.emit("if (%s == 0) %s[%s] = 1ull;   // marks the extra slot used" % (gtag, ftags, fcap))
                 pipe.code.close()
                 pipe.emit(overflow)
                 pipe.code.close()
@@ -1251,6 +1341,8 @@ class AggregationTranslator(UnaryTranslator):
         prog = ctxt.program
         name = "krnl_aggregation%d" % a.opId
         rows = "L.htcap[%d]" % (self.fused["htIndex"] if self.fused else self.ht["index"])
+        if self.fused is None and self.doGroup and self.exact64:
+            rows += " + 1"
         pipe = prog.newPipeline(name, {"rows": rows, "tidName": "agg%d" % a.opId}, rowsPerThread(self))
         ctxt.pipe = pipe
         tid = pipe.tid + "[u]"
@@ -1282,11 +1374,19 @@ class AggregationTranslator(UnaryTranslator):
             ftags = pipe.param("aht%d" % a.opId, "const uint64_t*", "B_aht%d_tags" % a.opId)
             gtag = pipe.reg("gtag", CType.UINT64, "0")
             with pipe.rowLoop():
-                pipe.emit("if (active[u]) { %s = %s[%s]; active[u] = (%s >> 63) ? 1 : 0; }" % (gtag, ftags, tid, gtag))
+                if self.exact64:
+                    fcap = pipe.param("aht%d_cap" % a.opId, "uint32_t", "L.htcap[%d]" % self.ht["index"])
+                    pipe.emit("if (active[u]) { %s = %s[%s]; active[u] = %s != 0 ? 1 : 0; }" % (gtag, ftags, tid, gtag))
+                    pipe.emit("const uint64_t pk = %s == (int64_t)%s ? ~0ull : %s - 1ull;" % (tid, fcap, gtag))
+                    src = "pk"
+                else:
+                    pipe.emit("if (active[u]) { %s = %s[%s]; active[u] = (%s >> 63) ? 1 : 0; }"
+                              % (gtag, ftags, tid, gtag))
+                    src = gtag
                 if self.exact:
                     for g, unpack in zip(groups, self.unpack):
                         st = pipe.setComputed(g)
-                        pipe.emit("%s[u] = %s;" % (st.reg, unpack(gtag)))
+                        pipe.emit("%s[u] = %s;" % (st.reg, unpack(src)))
             if not self.exact:
                 for g, (b, ct), k in zip(groups, self.keyBuffers, range(len(groups))):
                     f = pipe.param("aht%d_k%d" % (a.opId, k), "const %s*" % ct, "B_" + b)
@@ -1342,32 +1442,47 @@ def _keyEq(attr, a, b):
     return "(%s == %s)" % (a, b)
 
 
+_KEY_MASK = {8: "0xffull", 32: "0xffffffffull", 64: "0xffffffffffffffffull"}
+_KEY_SRC = {8: "uint8_t", 32: "uint32_t", 64: "uint64_t"}
+
+
 def _packKeys(attrs):
-    """Exact 63-bit tag of a key tuple, or None when it does not pack.
-    Returns (tagExprFn(pipe), [unpackFn(tagVar)])."""
+    """Exact tag of a group-key tuple, or None when the keys need more than
+    64 bits.  Up to 63 bits: TAG_TOP | packed (never 0).  Exactly 64 bits:
+    packed + 1, which is 0 only for packed == ~0 (that key gets the extra
+    slot at index cap).  Returns (tagExprFn(pipe), [unpackFn(packedVar)],
+    is64); the unpack functions take the tag itself in the 63-bit form and
+    the packed value in the 64-bit form."""
     widths = []
     for a in attrs:
-        w = PACK_BITS.get(a.dataType)
+        w = _GROUP_BITS.get(a.dataType)
         if w is None:
             return None
         widths.append(w)
-    if sum(widths) > 63:
+    if sum(widths) > 64 or not widths:
         return None
+    is64 = sum(widths) == 64
 
     def tagExpr(pipe):
-        parts = ["dqc::TAG_TOP"]
+        parts = []
         shift = 0
         for a, w in zip(attrs, widths):
-            mask = "0xffffffffull" if w == 32 else "0xffull"
-            parts.append("(((uint64_t)(%s) & %s) << %d)" % (pipe.access(a), mask, shift))
+            parts.append("(((uint64_t)(%s) & %s) << %d)" % (pipe.access(a), _KEY_MASK[w], shift))
             shift += w
-        return " | ".join(parts)
+        if is64:
+            return "((%s) + 1ull)" % " | ".join(parts)
+        return " | ".join(["dqc::TAG_TOP"] + parts)
 
     unpacks = []
     shift = 0
     for a, w in zip(attrs, widths):
         ct = langType(a.dataType, CType.FP64)
-        src = "uint32_t" if w == 32 else "uint8_t"
-        unpacks.append(lambda tv, s=shift, src=src, ct=ct: "(%s)(%s)((%s) >> %d)" % (ct, src, tv, s))
+        unpacks.append(lambda tv, s=shift, src=_KEY_SRC[w], ct=ct: "(%s)(%s)((%s) >> %d)" % (ct, src, tv, s))
         shift += w
-    return tagExpr, unpacks
+    return tagExpr, unpacks, is64
+
+
+# bits of a group key in an exact aggregation tag (joins pack up to 63 bits
+# with PACK_BITS; INT64 keys fit only the 64-bit aggregation form)
+_GROUP_BITS = dict(PACK_BITS)
+_GROUP_BITS[Type.INT64] = 64

@@ -160,11 +160,46 @@ def _featurePlan(name, alg):
                                          scal.LargerExpr(_A("u_w"), _C("2.5", Type.DOUBLE)),
                                          alg.scan("u"), alg.scan("t")))
         return p, {3: {"multimatch": True}}
+    if name == "f_groupby_64":
+        # two INT keys = 64-bit exact tag; (-1, -1) is the key stored in the
+        # table's extra slot
+        p = alg.aggregation(["t_grp", "g2"], [(Reduction.SUM, "t_val", "s"), (Reduction.COUNT, "", "c"),
+                                               (Reduction.MIN, "t_id", "mi")],
+                            alg.map("g2", scal.MulExpr(_A("t_grp"), _C("1", Type.INT)), alg.scan("t")))
+        return p, {3: {"numgroups": 128}}
+    if name == "f_groupby_int64":
+        # one INT64 key (64-bit exact tag), about 2k groups
+        p = alg.aggregation(["t_bq"], [(Reduction.SUM, "t_f", "sf"), (Reduction.COUNT, "", "c")],
+                            alg.map("t_bq", scal.DivExpr(_A("t_big"), _C("1073741824", Type.INT64)), alg.scan("t")))
+        return p, {3: {"numgroups": 2100}}
+    if name == "f_temp":
+        # TPC-H Q15 shape (query/plans/tpch15.py): a temp table of per-group
+        # sums, the maximum over one scan of it joined back to a second scan,
+        # then to the dimension
+        p = [alg.createtemp("tsum", alg.aggregation(["t_grp"], [(Reduction.SUM, "t_val", "sv"),
+                                                               (Reduction.COUNT, "", "c")], alg.scan("t"))),
+             alg.projection(["u_id", "u_name", "t_grp", "c", "best"],
+                            alg.join(("t_grp", "u_grp"),
+                                     alg.join(("best", "sv"),
+                                              alg.aggregation([], [(Reduction.MAX, "sv", "best")],
+                                                              alg.scan("tsum", "r1")),
+                                              alg.scan("tsum", "r2")),
+                                     alg.scan("u")))]
+        return p, {2: {"numgroups": 100}, 7: {"multimatch": False}, 9: {"multimatch": False}}
+    if name == "f_temp_strings":
+        # strings and a filter through a temp table, grouped by the string
+        p = [alg.createtemp("tlike", alg.projection(["t_id", "t_txt", "t_val"],
+                                                    alg.selection(scal.LikeExpr(_A("t_txt"), _C("%golf%", Type.STRING)),
+                                                                  alg.scan("t")))),
+             alg.aggregation(["t_txt"], [(Reduction.SUM, "t_val", "s"), (Reduction.COUNT, "", "c")],
+                             alg.selection(scal.LargerExpr(_A("t_id"), _C("10", Type.INT)), alg.scan("tlike")))]
+        return p, {2: {"selectivity": 0.05}, 7: {"numgroups": 64}}
     raise KeyError(name)
 
 
 FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
-                 "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi"]
+                 "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
+                 "f_groupby_int64"]
 
 # name -> (workload or "feature", options)
 PLANS = OrderedDict()
@@ -174,6 +209,7 @@ PLANS["c2_none"] = ("c2", {})
 PLANS["c2_refill"] = ("c2", {"buffers": [3]})
 PLANS["c2_pushdown"] = ("c2", {"pushdown": True})
 PLANS["c2_both"] = ("c2", {"buffers": [3], "pushdown": True})
+PLANS["c3"] = ("c3", {})
 PLANS["cal"] = ("cal", {})
 for _n in FEATURE_PLANS:
     PLANS[_n] = ("feature", {})

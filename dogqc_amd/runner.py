@@ -38,6 +38,27 @@ def resultColumns(module, execution, accessor):
     return out
 
 
+def tempColumns(module, execution, accessor, name):
+    """A temp table's columns, strings decoded to bytes (their device
+    pointers are turned into offsets of the bound char heap)."""
+    t = execution._temp(name)
+    cols = execution.temp(name)
+    byIndex = {c["index"]: c for c in module.desc["columns"]}
+    out = OrderedDict()
+    for c in t["columns"]:
+        arr = cols[c["name"]]
+        if c["ctype"] == "dqc::str_t":
+            base = int(execution.inputs.ptrs[c["char_col"]])
+            offs = np.zeros(len(arr), dtype=[("start", np.uint64), ("end", np.uint64)])
+            offs["start"] = arr["start"] - np.uint64(base)
+            offs["end"] = arr["end"] - np.uint64(base)
+            src = byIndex[c["char_col"]]
+            out[c["name"]] = decodeStrings(offs, accessor.column(src["table"], src["attr"])[1])
+        else:
+            out[c["name"]] = arr
+    return out
+
+
 def timeSteps(module, inputs, tuning, stream, ws):
     """Per-launch milliseconds of one enqueue (events between launches)."""
     steps = module.steps()

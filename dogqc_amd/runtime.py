@@ -15,7 +15,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(PKG, "libdqcrt.so")
 
-MAX_PIPES, MAX_HTS, MAX_RESULT_COLS, PROFILE_BUCKETS = 32, 32, 64, 65
+MAX_PIPES, MAX_HTS, MAX_RESULT_COLS, MAX_TEMPS, PROFILE_BUCKETS = 32, 32, 64, 8, 65
 
 RUNTIME_SYMBOLS = [
     "dqc_last_error", "dqc_device_init", "dqc_malloc", "dqc_free", "dqc_memcpy_h2d", "dqc_memcpy_d2h",
@@ -37,19 +37,22 @@ class Tuning(C.Structure):
                 ("lds_slots", C.c_int32 * MAX_PIPES), ("lds_replicas", C.c_int32 * MAX_PIPES),
                 ("ht_factor", C.c_float * MAX_HTS), ("ht_capacity", C.c_int64 * MAX_HTS),
                 ("mm_entries", C.c_int64 * MAX_HTS), ("result_capacity", C.c_int64),
-                ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("temp_capacity", C.c_int64 * MAX_TEMPS), ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class Status(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("ht_overflow_mask", C.c_uint32), ("mm_overflow_mask", C.c_uint32),
-                ("reserved", C.c_uint32), ("result_count", C.c_uint64), ("mm_total", C.c_uint64 * MAX_HTS)]
+                ("reserved", C.c_uint32), ("result_count", C.c_uint64), ("mm_total", C.c_uint64 * MAX_HTS),
+                ("temp_count", C.c_uint64 * MAX_TEMPS)]
 
 
 class Layout(C.Structure):
     _fields_ = [("ws_bytes", C.c_uint64), ("nsteps", C.c_int32), ("nhts", C.c_int32),
                 ("ht_capacity", C.c_int64 * MAX_HTS), ("mm_entries", C.c_int64 * MAX_HTS),
                 ("result_capacity", C.c_int64), ("nresult", C.c_int32), ("nprofile", C.c_int32),
-                ("result_offset", C.c_uint64 * MAX_RESULT_COLS), ("profile_offset", C.c_uint64)]
+                ("result_offset", C.c_uint64 * MAX_RESULT_COLS), ("profile_offset", C.c_uint64),
+                ("ntemps", C.c_int32), ("ntempcols", C.c_int32), ("temp_capacity", C.c_int64 * MAX_TEMPS),
+                ("temp_offset", C.c_uint64 * MAX_RESULT_COLS)]
 
 
 class QueryResult(C.Structure):
@@ -208,7 +211,8 @@ class Event(object):
 
 CTYPE_DTYPE = {"int32_t": np.int32, "uint32_t": np.uint32, "int64_t": np.int64, "uint64_t": np.uint64,
                "float": np.float32, "double": np.float64, "int8_t": np.int8, "char": np.uint8,
-               "dqc::str_offs": np.dtype([("start", np.uint64), ("end", np.uint64)])}
+               "dqc::str_offs": np.dtype([("start", np.uint64), ("end", np.uint64)]),
+               "dqc::str_t": np.dtype([("start", np.uint64), ("end", np.uint64)])}
 
 
 class PlanModule(object):
@@ -269,6 +273,7 @@ class Execution(object):
 
     def __init__(self, module, inputs, tuning, res):
         self.module, self.inputs, self.tuning, self.res = module, inputs, tuning, res
+        self.owned = True
 
     @property
     def count(self):
@@ -283,6 +288,29 @@ class Execution(object):
             out[c["name"]] = copyToHost(self.res.ws + self.res.layout.result_offset[i], dt, n)
         return out
 
+    def tempCount(self, name):
+        t = self._temp(name)
+        return int(self.res.status.temp_count[t["index"]])
+
+    def _temp(self, name):
+        for t in self.module.desc["temps"]:
+            if t["name"] == name:
+                return t
+        raise KeyError("no temp table %s in this plan" % name)
+
+    def temp(self, name):
+        """Columns of a temp table (createtemp) as an OrderedDict name ->
+        numpy array.  String columns hold device (start, end) pointers into
+        the input char heap."""
+        t = self._temp(name)
+        lay = self.res.layout
+        n = min(self.tempCount(name), int(lay.temp_capacity[t["index"]]))
+        out = OrderedDict()
+        for c in t["columns"]:
+            dt = CTYPE_DTYPE[c["ctype"]]
+            out[c["name"]] = copyToHost(self.res.ws + lay.temp_offset[c["slot"]], dt, n)
+        return out
+
     def profile(self):
         lay = self.res.layout
         if lay.nprofile == 0:
@@ -292,14 +320,53 @@ class Execution(object):
                 for i, p in enumerate(self.module.desc["profiles"])}
 
     def release(self):
-        if self.res.ws:
+        if self.res.ws and self.owned:
             lib().dqc_query_release(C.byref(self.res))
+        self.res.ws = None
 
     def __del__(self):
         try:
             self.release()
         except Exception:
             pass
+
+
+class Graph(object):
+    """One plan enqueue captured into an executable hipGraph: every launch
+    of the plan replays the whole kernel sequence with one call."""
+
+    def __init__(self, module, inputs, tuning, ws, stream):
+        self.handle = C.c_void_p()
+        check(lib().dqc_graph_create(module.handle, C.byref(inputs.struct), C.byref(tuning), ws.ptr, ws.nbytes,
+                                     stream.handle, C.byref(self.handle)), "dqc_graph_create")
+
+    def launch(self, stream):
+        check(lib().dqc_graph_launch(self.handle, stream.handle), "dqc_graph_launch")
+
+    def destroy(self):
+        if self.handle:
+            lib().dqc_graph_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def view(module, inputs, tuning, ws):
+    """An Execution over a caller-owned workspace after an enqueue or graph
+    launch has completed (the status is read from the workspace now; the
+    workspace is not freed by the view)."""
+    res = QueryResult()
+    res.ws, res.ws_bytes = ws.ptr.value, ws.nbytes
+    res.layout = layout(module, inputs, tuning)
+    st = ws.toArray(np.uint8, C.sizeof(Status))
+    C.memmove(C.addressof(res.status), st.ctypes.data, C.sizeof(Status))
+    ex = Execution(module, inputs, tuning, res)
+    ex.owned = False
+    return ex
 
 
 def run(module, inputs, tuning=None, stream=None, max_retries=8):

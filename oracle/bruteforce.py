@@ -24,6 +24,7 @@ class BruteForce(object):
         self.acc = accessor
         self.decimal = decimal
         self.scalar = Evaluator(accessor, decimal)
+        self.temps = {}      # temp table name -> list of {attribute name: value}
 
     def value(self, e, row):
         """Evaluate an expression on one tuple (a dict attr id -> value) by
@@ -34,6 +35,9 @@ class BruteForce(object):
         return v.reshape(-1)[0] if v.ndim else v[()]
 
     def rows(self, node):
+        if isinstance(node, ra.Scan) and node.isTempScan:
+            byName = {a.name: id for id, a in node.outRelation.items()}
+            return [{byName[k]: v for k, v in r.items() if k in byName} for r in self.temps[node.table["name"]]]
         if isinstance(node, ra.Scan):
             n = node.table["size"]
             cols = {}
@@ -58,6 +62,10 @@ class BruteForce(object):
                 r[node.mapAttr.id] = v if ct == "str" else np.array(v).astype(ct)[()]
                 out.append(r)
             return out
+        if isinstance(node, ra.Materialize) and node.matType == ra.MaterializationType.TEMPTABLE:
+            rows = self.rows(node.child)
+            self.temps[node.table["name"]] = [{a.name: r[id] for id, a in node.outRelation.items()} for r in rows]
+            return rows
         if isinstance(node, (ra.Projection, ra.Materialize)):
             return self.rows(node.child)
         if isinstance(node, ra.EquiJoin):

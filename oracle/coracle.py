@@ -67,3 +67,32 @@ def c2(build, probe, fmax=10, threads=8):
                              C.c_int32(fmax), C.c_int(threads), C.byref(cnt), C.byref(s))
     assert rc == 0
     return cnt.value, s.value
+
+
+def c3(tables, word, threads=8):
+    """(by_item, by_nation_year) of the c3 plan, each an OrderedDict of
+    columns in the plan's output order."""
+    from collections import OrderedDict
+    item, cust, nation, sales = tables["item"], tables["cust"], tables["nation"], tables["sales"]
+    ik = np.ascontiguousarray(item["i_key"])
+    offs, chars = item["i_text"]
+    offs, chars = np.ascontiguousarray(offs, dtype=np.uint64), np.ascontiguousarray(chars, dtype=np.uint8)
+    ck, cy = np.ascontiguousarray(cust["c_key"]), np.ascontiguousarray(cust["c_year"])
+    nk = np.ascontiguousarray(nation["n_key"])
+    s = [np.ascontiguousarray(sales[k]) for k in ("s_item", "s_cust", "s_nation", "s_qty", "s_price")]
+    ci = len(ik) + 1
+    cn = max(1, len(nk)) * 64 + 1
+    out_i = [np.zeros(ci, dtype=t) for t in (np.int32, np.float64, np.float64, np.int32)]
+    out_n = [np.zeros(cn, dtype=t) for t in (np.int32, np.int32, np.float64, np.float64, np.int32)]
+    nny = C.c_int64()
+    L = lib()
+    L.dqc_oracle_c3.restype = C.c_int64
+    n = L.dqc_oracle_c3(_p(ik), _p(offs), _p(chars), C.c_int64(len(ik)), C.c_char_p(word.encode()),
+                        _p(ck), _p(cy), C.c_int64(len(ck)), _p(nk), C.c_int64(len(nk)),
+                        *[_p(a) for a in s], C.c_int64(len(s[0])), C.c_int(threads),
+                        *[_p(a) for a in out_i], C.c_int64(ci), *[_p(a) for a in out_n], C.c_int64(cn),
+                        C.byref(nny))
+    assert 0 <= n <= ci and 0 <= nny.value <= cn
+    byItem = OrderedDict(zip(["i_key", "sum_qty", "revenue", "cnt"], [a[:n] for a in out_i]))
+    byNy = OrderedDict(zip(["n_key", "c_year", "sum_qty", "revenue", "cnt"], [a[:nny.value] for a in out_n]))
+    return byItem, byNy

@@ -1,5 +1,5 @@
 /*
- * Multithreaded C oracle of the three benchmark workloads.
+ * Multithreaded C oracle of the four benchmark workloads.
  *
  * TEST INFRASTRUCTURE ONLY: the CPU-side checker and the CPU baseline of the
  * benchmark.  Nothing on the product path links or calls it.
@@ -366,4 +366,206 @@ int dqc_oracle_c2(const int32_t* b_key, const int64_t* b_pay, int64_t nb, const 
     free(m.ekey);
     free(m.epay);
     return 0;
+}
+
+/* ------------------------------------------------------------------ c3 */
+/* item: LIKE '%word%' on a STRING column (offsets + heap): for a pattern
+ * whose only wildcards are a leading and a trailing '%', SQL LIKE is a
+ * substring test (util.cuh:144-199 reduces to that case).  sales probes
+ * item (unique, filtered), cust (unique), nation (unique) and every joined
+ * row is aggregated twice: by (n_key, c_year) and by i_key, each with
+ * SUM(s_qty) (exact in int64, a double aggregate in the plan), SUM(s_price)
+ * (compensated) and COUNT. */
+typedef struct {
+    int64_t cap, n;
+    int64_t* keys;
+    int64_t* vals;
+    uint8_t* used;
+} i64map;
+
+static int i64map_init(i64map* m, int64_t expect) {
+    int64_t cap = 64;
+    while (cap < 2 * expect + 1) cap *= 2;
+    m->cap = cap;
+    m->n = 0;
+    m->keys = calloc((size_t)cap, sizeof(int64_t));
+    m->vals = calloc((size_t)cap, sizeof(int64_t));
+    m->used = calloc((size_t)cap, 1);
+    return (m->keys && m->vals && m->used) ? 0 : -1;
+}
+
+static void i64map_free(i64map* m) {
+    free(m->keys);
+    free(m->vals);
+    free(m->used);
+}
+
+/* slot of key k: existing, or claimed (used=1, vals=-1) when insert != 0, else -1 */
+static int64_t i64map_slot(i64map* m, int64_t k, int insert) {
+    uint64_t s = hmix((uint64_t)k) & (uint64_t)(m->cap - 1);
+    for (;;) {
+        if (!m->used[s]) {
+            if (!insert) return -1;
+            m->used[s] = 1;
+            m->keys[s] = k;
+            m->vals[s] = -1;
+            m->n++;
+            return (int64_t)s;
+        }
+        if (m->keys[s] == k) return (int64_t)s;
+        s = (s + 1) & (uint64_t)(m->cap - 1);
+    }
+}
+
+typedef struct {
+    int64_t qty, cnt;
+    ksum rev;
+} c3_acc;
+
+typedef struct {
+    const int32_t *s_item, *s_cust, *s_nation, *s_qty;
+    const double* s_price;
+    const i64map *items, *custs, *nations;
+    int64_t nitems_pass, ngroups_ny;
+    c3_acc** by_item;   /* per thread: nitems_pass */
+    c3_acc** by_ny;     /* per thread: ngroups_ny = nations x years (dense) */
+    int32_t year_lo;
+    int32_t nyears;
+} c3_arg;
+
+static void c3_part(void* p, int t, int64_t lo, int64_t hi) {
+    c3_arg* a = (c3_arg*)p;
+    c3_acc* bi = a->by_item[t];
+    c3_acc* bn = a->by_ny[t];
+    for (int64_t i = lo; i < hi; ++i) {
+        int64_t si = i64map_slot((i64map*)a->items, a->s_item[i], 0);
+        if (si < 0) continue;
+        int64_t sc = i64map_slot((i64map*)a->custs, a->s_cust[i], 0);
+        if (sc < 0) continue;
+        int64_t sn = i64map_slot((i64map*)a->nations, a->s_nation[i], 0);
+        if (sn < 0) continue;
+        int64_t q = a->s_qty[i];
+        double pr = a->s_price[i];
+        c3_acc* x = &bi[a->items->vals[si]];
+        x->qty += q;
+        x->cnt += 1;
+        kadd(&x->rev, pr);
+        int64_t g = a->nations->vals[sn] * a->nyears + (a->custs->vals[sc] - a->year_lo);
+        c3_acc* y = &bn[g];
+        y->qty += q;
+        y->cnt += 1;
+        kadd(&y->rev, pr);
+    }
+}
+
+static int contains(const char* s, int64_t n, const char* w, int64_t wn) {
+    if (wn == 0) return 1;
+    for (int64_t i = 0; i + wn <= n; ++i)
+        if (s[i] == w[0] && memcmp(s + i, w, (size_t)wn) == 0) return 1;
+    return 0;
+}
+
+/* Outputs (capacities from the caller):
+ *   item groups:  out_item_key / _qty / _rev / _cnt, at most cap_item rows;
+ *   (n_key, c_year) groups: out_ny_nkey / _year / _qty / _rev / _cnt, at most cap_ny rows.
+ * Returns the number of item groups (and *n_ny), or -1 on error. */
+int64_t dqc_oracle_c3(const int32_t* i_key, const uint64_t* i_offs, const char* i_chars, int64_t ni, const char* word,
+                      const int32_t* c_key, const int32_t* c_year, int64_t nc, const int32_t* n_key, int64_t nn,
+                      const int32_t* s_item, const int32_t* s_cust, const int32_t* s_nation, const int32_t* s_qty,
+                      const double* s_price, int64_t ns, int nthreads, int32_t* out_item_key, double* out_item_qty,
+                      double* out_item_rev, int32_t* out_item_cnt, int64_t cap_item, int32_t* out_ny_nkey,
+                      int32_t* out_ny_year, double* out_ny_qty, double* out_ny_rev, int32_t* out_ny_cnt,
+                      int64_t cap_ny, int64_t* n_ny) {
+    if (nthreads < 1) nthreads = 1;
+    const int64_t wn = (int64_t)strlen(word);
+    i64map items, custs, nations;
+    if (i64map_init(&items, ni / 8 + 16) || i64map_init(&custs, nc) || i64map_init(&nations, nn)) return -1;
+    int64_t npass = 0;
+    for (int64_t i = 0; i < ni; ++i) {
+        if (!contains(i_chars + i_offs[i], (int64_t)(i_offs[i + 1] - i_offs[i]), word, wn)) continue;
+        if (items.n * 2 + 2 > items.cap) {   /* grow */
+            i64map bigger;
+            if (i64map_init(&bigger, items.cap)) return -1;
+            for (int64_t s = 0; s < items.cap; ++s)
+                if (items.used[s]) bigger.vals[i64map_slot(&bigger, items.keys[s], 1)] = items.vals[s];
+            i64map_free(&items);
+            items = bigger;
+        }
+        int64_t s = i64map_slot(&items, i_key[i], 1);
+        if (items.vals[s] < 0) items.vals[s] = npass++;
+    }
+    int32_t ylo = 0, yhi = -1;
+    for (int64_t i = 0; i < nc; ++i) {
+        if (i == 0 || c_year[i] < ylo) ylo = c_year[i];
+        if (i == 0 || c_year[i] > yhi) yhi = c_year[i];
+        custs.vals[i64map_slot(&custs, c_key[i], 1)] = c_year[i];
+    }
+    const int32_t nyears = nc > 0 ? yhi - ylo + 1 : 1;
+    int64_t nnat = 0;
+    for (int64_t i = 0; i < nn; ++i) {
+        int64_t s = i64map_slot(&nations, n_key[i], 1);
+        if (nations.vals[s] < 0) nations.vals[s] = nnat++;
+    }
+    const int64_t ngroups = (nnat > 0 ? nnat : 1) * (int64_t)nyears;
+    c3_arg a = {s_item, s_cust, s_nation, s_qty, s_price, &items, &custs, &nations, npass, ngroups, NULL, NULL,
+                ylo, nyears};
+    a.by_item = calloc((size_t)nthreads, sizeof(c3_acc*));
+    a.by_ny = calloc((size_t)nthreads, sizeof(c3_acc*));
+    for (int t = 0; t < nthreads; ++t) {
+        a.by_item[t] = calloc((size_t)(npass + 1), sizeof(c3_acc));
+        a.by_ny[t] = calloc((size_t)ngroups, sizeof(c3_acc));
+    }
+    parallel_for(nthreads, ns, &a, c3_part);
+    for (int t = 1; t < nthreads; ++t) {
+        for (int64_t g = 0; g < npass; ++g) {
+            a.by_item[0][g].qty += a.by_item[t][g].qty;
+            a.by_item[0][g].cnt += a.by_item[t][g].cnt;
+            kmerge(&a.by_item[0][g].rev, &a.by_item[t][g].rev);
+        }
+        for (int64_t g = 0; g < ngroups; ++g) {
+            a.by_ny[0][g].qty += a.by_ny[t][g].qty;
+            a.by_ny[0][g].cnt += a.by_ny[t][g].cnt;
+            kmerge(&a.by_ny[0][g].rev, &a.by_ny[t][g].rev);
+        }
+    }
+    int64_t nout = 0;
+    for (int64_t s = 0; s < items.cap; ++s) {
+        if (!items.used[s]) continue;
+        const c3_acc* x = &a.by_item[0][items.vals[s]];
+        if (x->cnt == 0) continue;
+        if (nout < cap_item) {
+            out_item_key[nout] = (int32_t)items.keys[s];
+            out_item_qty[nout] = (double)x->qty;
+            out_item_rev[nout] = kval(&x->rev);
+            out_item_cnt[nout] = (int32_t)x->cnt;
+        }
+        ++nout;
+    }
+    int64_t nny = 0;
+    for (int64_t s = 0; s < nations.cap; ++s) {
+        if (!nations.used[s]) continue;
+        for (int32_t y = 0; y < nyears; ++y) {
+            const c3_acc* x = &a.by_ny[0][nations.vals[s] * nyears + y];
+            if (x->cnt == 0) continue;
+            if (nny < cap_ny) {
+                out_ny_nkey[nny] = (int32_t)nations.keys[s];
+                out_ny_year[nny] = ylo + y;
+                out_ny_qty[nny] = (double)x->qty;
+                out_ny_rev[nny] = kval(&x->rev);
+                out_ny_cnt[nny] = (int32_t)x->cnt;
+            }
+            ++nny;
+        }
+    }
+    *n_ny = nny;
+    for (int t = 0; t < nthreads; ++t) {
+        free(a.by_item[t]);
+        free(a.by_ny[t]);
+    }
+    free(a.by_item);
+    free(a.by_ny);
+    i64map_free(&items);
+    i64map_free(&custs);
+    i64map_free(&nations);
+    return nout;
 }

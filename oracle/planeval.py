@@ -20,6 +20,8 @@ It restates the reference's operator semantics operator-at-a-time:
     error is below one ulp), integer sums are exact; COUNT is int32; AVG =
     SUM / COUNT; without grouping exactly one row is produced; MIN starts at
     the type's maximum and MAX at its lowest value;
+  * a temp table (createtemp, relationalAlgebra.py Materialize.temp) holds
+    its child's rows; a later root's scan of it reads them by attribute name;
   * SQL LIKE follows util.cuh:144-199 for '%' and '_' patterns (via an
     equivalent regular expression), SUBSTRING util.cuh:132-142.
 The result is an OrderedDict attribute name -> numpy array (strings as
@@ -120,6 +122,7 @@ class Evaluator(object):
     def __init__(self, accessor, decimal=FP64):
         self.acc = accessor
         self.decimal = decimal
+        self.temps = {}      # temp table name -> {attribute name: column}
 
     # ------------------------------------------------------------ scalars
     def expr(self, e, rel, n):
@@ -243,6 +246,11 @@ class Evaluator(object):
         tname = node.table["name"]
         rel = {}
         n = None
+        if node.isTempScan:
+            cols, n = self.temps[tname]
+            for id, a in node.outRelation.items():
+                rel[id] = cols[a.name]
+            return rel, n
         for id, a in node.outRelation.items():
             data = self.acc.column(tname, a.name)
             if a.dataType == Type.STRING:
@@ -283,9 +291,12 @@ class Evaluator(object):
         return rel, n
 
     def _Materialize(self, node):
-        if node.matType != ra.MaterializationType.RESULT:
-            raise NotImplementedError("temp tables")
-        return self.node(node.child)
+        """Result, or a temp table (relationalAlgebra.py createtemp): its
+        columns become scannable by name for the plan's later roots."""
+        rel, n = self.node(node.child)
+        if node.matType == ra.MaterializationType.TEMPTABLE:
+            self.temps[node.table["name"]] = ({a.name: rel[id] for id, a in node.outRelation.items()}, n)
+        return rel, n
 
     @staticmethod
     def _codes(cols):

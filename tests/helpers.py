@@ -11,19 +11,33 @@ from dogqc_amd.relationalAlgebra import Reduction
 
 
 def toleranceColumns(plan):
-    """Names of floating-point SUM/AVG aggregate outputs in a resolved plan."""
-    names = set()
+    """Names of the result columns that carry a floating-point SUM/AVG
+    aggregate: the aggregate outputs themselves and anything derived from
+    them (MIN/MAX over them, maps of them, their temp-table copies)."""
+    import dogqc_amd.scalarAlgebra as scal
+    tol = set()
+
+    def uses(expr):
+        if isinstance(expr, scal.AttrExpr):
+            return expr.attr.id in tol
+        return any(uses(c) for c in expr.children())
 
     def walk(node):
-        if isinstance(node, ra.Aggregation):
-            for id, (inId, red) in node.aggregateTuples.items():
-                if red in (Reduction.SUM, Reduction.AVG):
-                    names.add(node.aggregateAttributes[id].name)
         for c in node.children():
             walk(c)
+        if isinstance(node, ra.Scan) and node.isTempScan:
+            for id, a in node.outRelation.items():
+                if getattr(a, "sourceId", None) in tol:
+                    tol.add(id)
+        elif isinstance(node, ra.Map) and uses(node.expression):
+            tol.add(node.mapAttr.id)
+        elif isinstance(node, ra.Aggregation):
+            for id, (inId, red) in node.aggregateTuples.items():
+                if red in (Reduction.SUM, Reduction.AVG) or (red in (Reduction.MIN, Reduction.MAX) and inId in tol):
+                    tol.add(id)
     for root in plan:
         walk(root)
-    return names
+    return {a.name for id, a in plan[-1].outRelation.items() if id in tol}
 
 
 def _sortKey(cols, names):

@@ -152,6 +152,99 @@ def test_c2_full_closed_form(variant):
     assert int(got["sum_prod"][0]) == s
 
 
+# ------------------------------------------------------------ c3 star join
+
+C3_SMALL = {"sales": 200000, "item": 20000, "cust": 2000, "nation": 25}
+
+
+def c3Run(acc, tuning=None, storeKey=None):
+    mod = module("c3")
+    store = _STORE.setdefault(storeKey, {}) if storeKey else {}
+    inputs = rt.BoundInputs(mod, acc, store=store)
+    ex = rt.run(mod, inputs, tuning or rt.newTuning())
+    got = runner.resultColumns(mod, ex, acc)
+    ny = runner.tempColumns(mod, ex, acc, "c3_by_nation_year")
+    retries = ex.res.retries
+    ex.release()
+    return got, ny, retries
+
+
+def c3Oracle(acc):
+    alg, p, cfg, _ = planlib.build("c3", acc)
+    plan = alg.resolveAlgebraPlan(p, cfg)
+    from oracle.planeval import Evaluator
+    ev = Evaluator(acc)
+    res = ev.run(plan)
+    return res, ev.temps["c3_by_nation_year"][0]
+
+
+@pytest.mark.parametrize("rows", [C3_SMALL, {"sales": 1, "item": 1, "cust": 1, "nation": 1},
+                                  {"sales": 0, "item": 100, "cust": 10, "nation": 25},
+                                  {"sales": 5000, "item": 0, "cust": 10, "nation": 25},
+                                  {"sales": 70001, "item": 3000, "cust": 300, "nation": 5}])
+def test_c3_small(rows):
+    print("gpu case: c3 %r" % rows, flush=True)
+    acc = workloadAcc("c3", rows, seed=41)
+    got, ny, _ = c3Run(acc)
+    want, wantNy = c3Oracle(acc)
+    assertSameResult(got, want, {"revenue"}, label="c3 by item")
+    assertSameResult(ny, wantNy, {"revenue"}, label="c3 by nation, year")
+
+
+def test_c3_temp_overflow_retry():
+    print("gpu case: c3 undersized temp tables", flush=True)
+    acc = workloadAcc("c3", C3_SMALL, seed=42)
+    t = rt.newTuning()
+    t.temp_capacity[0] = 10
+    t.temp_capacity[1] = 1
+    got, ny, retries = c3Run(acc, tuning=t)
+    assert retries > 0
+    want, wantNy = c3Oracle(acc)
+    assertSameResult(got, want, {"revenue"}, label="c3 retry by item")
+    assertSameResult(ny, wantNy, {"revenue"}, label="c3 retry by nation, year")
+
+
+def test_c3_full_vs_c_oracle():
+    from oracle import coracle
+    print("gpu case: c3 full 60M x 2M x 200k x 25", flush=True)
+    data = workloads.generate("c3")
+    acc = MemoryAccessor(data)
+    got, ny, _ = c3Run(acc)
+    tables = {t: {a: v[1] for a, v in cols.items()} for t, cols in data.items()}
+    wantItem, wantNy = coracle.c3(tables, workloads.C3_WORD)
+    assert len(wantItem["i_key"]) > 90000 and len(wantNy["n_key"]) == 175
+    assertSameResult(got, wantItem, {"revenue"}, label="c3 full by item")
+    assertSameResult(ny, wantNy, {"revenue"}, label="c3 full by nation, year")
+
+
+def test_c3_graph_replays_match_eager():
+    """The whole c3 plan captured into one hipGraph, replayed three times
+    with host reads of the workspace in between: every replay equals the
+    eager run."""
+    print("gpu case: c3 graph replays", flush=True)
+    acc = workloadAcc("c3", C3_SMALL, seed=43)
+    mod = module("c3")
+    inputs = rt.BoundInputs(mod, acc)
+    t = rt.newTuning()
+    ex = rt.run(mod, inputs, t)       # sizes the tables (tuning updated on retries)
+    eager = runner.resultColumns(mod, ex, acc)
+    eagerNy = runner.tempColumns(mod, ex, acc, "c3_by_nation_year")
+    ex.release()
+    lay = rt.layout(mod, inputs, t)
+    ws = rt.DeviceBuffer(lay.ws_bytes)
+    stream = rt.Stream()
+    g = rt.Graph(mod, inputs, t, ws, stream)
+    for k in range(3):
+        g.launch(stream)
+        stream.sync()
+        v = rt.view(mod, inputs, t, ws)
+        assertSameResult(runner.resultColumns(mod, v, acc), eager, {"revenue"}, rtol=1e-12,
+                         label="replay %d" % k)
+        assertSameResult(runner.tempColumns(mod, v, acc, "c3_by_nation_year"), eagerNy, {"revenue"}, rtol=1e-12,
+                         label="replay %d temp" % k)
+    g.destroy()
+
+
 # ------------------------------------------------------------ features
 
 @pytest.mark.parametrize("name", [n for n in planlib.PLANS if planlib.PLANS[n][0] == "feature"])

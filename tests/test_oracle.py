@@ -6,7 +6,8 @@ checker of the HIP path:
   * its scalar arithmetic against gcc compiling the same C expressions
     (the reference's generated code is C, so C decides the semantics);
   * against closed forms: per-group counts and integer sums by bincount, the
-    expanding join's COUNT and SUM(b*p) factorised per key;
+    expanding join's COUNT and SUM(b*p) factorised per key, the star join's
+    two aggregations by inverse-permutation indexing;
   * against the independent multithreaded C oracle at full workload size;
   * a mistake planted in the oracle (floor instead of truncating division)
     must fail the gcc check.
@@ -14,6 +15,7 @@ checker of the HIP path:
 import os
 import subprocess
 import tempfile
+from collections import OrderedDict
 
 import numpy as np
 import pytest
@@ -26,7 +28,8 @@ from dogqc_amd.types import Type
 from oracle import planeval, bruteforce, coracle
 from tests.helpers import assertSameResult, toleranceColumns
 
-SMALL = {"c0": {"cfg0": 400}, "c1": {"dim": 120, "fact": 900}, "c2": {"build": 150, "probe": 3000}}
+SMALL = {"c0": {"cfg0": 400}, "c1": {"dim": 120, "fact": 900}, "c2": {"build": 150, "probe": 3000},
+         "c3": {"sales": 1500, "item": 300, "cust": 40, "nation": 5}}
 
 
 def resolved(name, acc):
@@ -222,6 +225,66 @@ def test_c2_closed_form():
     from tests.test_gpu_parity import c2ClosedForm
     cnt, s = c2ClosedForm(acc)
     assert int(res["cnt"][0]) == cnt and int(res["sum_prod"][0]) == s
+
+
+def c3ClosedForm(g):
+    """Both c3 aggregations by direct indexing (every dimension key is a
+    permutation of 0..N-1, so key -> row is an inverse permutation) and
+    bincount; the LIKE test as a Python substring test."""
+    it, cu, na, sa = (g[t] for t in ("item", "cust", "nation", "sales"))
+    offs, chars = it["i_text"][1]
+    b = chars.tobytes()
+    word = workloads.C3_WORD.encode()
+    passing = np.array([word in b[int(offs[i]):int(offs[i + 1])] for i in range(len(offs) - 1)], dtype=bool)
+    passByKey = np.zeros(len(passing), dtype=bool)
+    passByKey[it["i_key"][1]] = passing
+    yearByKey = np.zeros(len(cu["c_key"][1]), dtype=np.int64)
+    yearByKey[cu["c_key"][1]] = cu["c_year"][1]
+    si, sc, sn = (sa[k][1].astype(np.int64) for k in ("s_item", "s_cust", "s_nation"))
+    keep = passByKey[si]
+    qty, price = sa["s_qty"][1][keep].astype(np.float64), sa["s_price"][1][keep]
+    si, sc, sn = si[keep], sc[keep], sn[keep]
+    cntItem = np.bincount(si, minlength=len(passing))
+    qtyItem = np.bincount(si, weights=qty, minlength=len(passing))
+    ny = sn * 10000 + yearByKey[sc]
+    keys, inv = np.unique(ny, return_inverse=True)
+    return (passing, cntItem, qtyItem, keys, np.bincount(inv), np.bincount(inv, weights=qty),
+            np.bincount(si, weights=price, minlength=len(passing)))
+
+
+def test_c3_closed_form():
+    g = workloads.generate("c3", {"sales": 40000, "item": 4000, "cust": 700, "nation": 25}, seed=10)
+    acc = MemoryAccessor(g)
+    passing, cntItem, qtyItem, nyKeys, nyCnt, nyQty, revItem = c3ClosedForm(g)
+    assert 0.03 < passing.mean() < 0.08      # 1 - (91/92)^5 = 5.3 %
+    alg, p, cfg, _ = planlib.build("c3", acc)
+    ev = planeval.Evaluator(acc)
+    res = ev.run(alg.resolveAlgebraPlan(p, cfg))
+    k = res["i_key"].astype(np.int64)
+    assert np.array_equal(np.sort(k), np.nonzero(cntItem)[0])
+    assert np.array_equal(res["cnt"], cntItem[k]) and np.array_equal(res["sum_qty"], qtyItem[k])
+    assert np.allclose(res["revenue"], revItem[k], rtol=1e-12, atol=0)
+    ny, _ = ev.temps["c3_by_nation_year"]
+    got = ny["n_key"].astype(np.int64) * 10000 + ny["c_year"]
+    o = np.argsort(got)
+    assert np.array_equal(got[o], nyKeys)
+    assert np.array_equal(ny["cnt"][o], nyCnt) and np.array_equal(ny["sum_qty"][o], nyQty)
+    byItem, byNy = coracle.c3({t: {a: v[1] for a, v in c.items()} for t, c in g.items()}, workloads.C3_WORD)
+    assertSameResult(byItem, res, {"revenue"}, rtol=1e-12, label="C oracle by item")
+    assertSameResult(byNy, OrderedDict((c, ny[c]) for c in byNy), {"revenue"}, rtol=1e-12, label="C oracle ny")
+
+
+def test_c3_text_column_layout():
+    assert len(workloads.VOCAB) == len(set(workloads.VOCAB)) == 92
+    assert not [(a, b) for a in workloads.VOCAB for b in workloads.VOCAB if a != b and a in b]
+    rng = np.random.Generator(np.random.PCG64(3))
+    idx = rng.integers(0, 92, size=(500, 5))
+    offs, chars = workloads.fixedWidthText(idx)
+    b = chars.tobytes()
+    assert np.array_equal(offs, np.arange(501, dtype=np.uint64) * 55) and len(b) == 500 * 55
+    for i in range(500):
+        row = b[55 * i:55 * (i + 1)].decode()
+        assert row.rstrip(" ").split(" ") == [workloads.VOCAB[j] for j in idx[i]]
 
 
 def test_empty_no_group_aggregate_yields_one_row():
