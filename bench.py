@@ -37,6 +37,9 @@ WORKLOADS = {
            "desc": "4-way star join + LIKE: sales 60M rows probe item 2M (LIKE '%green%' on 55-byte text, 5.3%), "
                    "cust 200k, nation 25; star join materialised once, grouped by (n_key, c_year) and by i_key "
                    "(SUM qty, SUM price, COUNT); whole plan replayed as one hipGraph"},
+    "c3_refill": {"table": "sales", "kernel": "krnl_sales5", "bytes_per_row": 4 + 4 + 4 + 4 + 8, "graph": True,
+                  "desc": "c3 with lane refill after the item join (survivors of several loop trips are "
+                          "gathered into full wavefronts before the cust/nation probes)"},
     "cal": {"table": "k", "kernel": "krnl_k1", "bytes_per_row": 8 + 8,
             "desc": "copy of one int64 column (8 B read + 8 B written per row): counter calibration"},
 }
@@ -74,7 +77,7 @@ def cpuBaseline(workload, g, seconds):
     elif workload.startswith("c2"):
         run = lambda: coracle.c2(cols["build"], cols["probe"], threads=threads)
         rows = len(cols["probe"]["p_key"])
-    elif workload == "c3":
+    elif workload.startswith("c3"):
         run = lambda: coracle.c3(cols, workloads.C3_WORD, threads=threads)
         rows = len(cols["sales"]["s_item"])
     else:
@@ -236,7 +239,7 @@ def checkParity(workload, g, got, temps):
     from oracle import coracle
     from dogqc_amd import workloads
     cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
-    if workload == "c3":
+    if workload.startswith("c3"):
         byItem, byNy = coracle.c3(cols, workloads.C3_WORD)
         ok = sameGroups(got, byItem, ["i_key"], {"revenue"}) and \
             sameGroups(temps["c3_by_nation_year"], byNy, ["n_key", "c_year"], {"revenue"})

@@ -175,6 +175,50 @@ DQC_DEVICE_INLINE int64_t ht_probe(const uint64_t* tags, uint32_t cap, uint64_t 
     return -1;
 }
 
+// The next PROBE_WINDOW slots s, s+1, ... of a unique join table, read with
+// two 16-byte loads so one memory round trip resolves most probes, hits and
+// misses alike.  Join tag buffers carry PROBE_WINDOW-1 padding slots so the
+// window never leaves the allocation; slots at or past cap are ignored.
+constexpr uint32_t PROBE_WINDOW = 4;
+
+struct probe_win {
+    uint64_t t[PROBE_WINDOW];
+};
+
+DQC_DEVICE_INLINE probe_win probe_window(const uint64_t* tags, uint32_t s) {
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2), aligned(8)));
+    const u64x2* q = (const u64x2*)(tags + s);
+    const u64x2 a = q[0], b = q[1];
+    probe_win w;
+    w.t[0] = a.x;
+    w.t[1] = a.y;
+    w.t[2] = b.x;
+    w.t[3] = b.y;
+    return w;
+}
+
+// Resolve a probe from its window: the slot holding `tag` (>= 0), -1 when an
+// empty slot comes first, -2 when every window slot holds another tag; then
+// the probe continues with ht_probe at `cursor` with `visited` slots tried.
+DQC_DEVICE_INLINE int64_t probe_in_window(const probe_win& w, uint32_t s, uint32_t cap, uint64_t tag,
+                                          uint32_t& cursor, uint32_t& visited) {
+    int64_t r = -2;
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < PROBE_WINDOW; ++i) {
+        if (r == -2 && s + i < cap) {
+            ++n;
+            if (w.t[i] == tag)
+                r = (int64_t)(s + i);
+            else if (w.t[i] == TAG_EMPTY)
+                r = -1;
+        }
+    }
+    visited = n;
+    cursor = s + n >= cap ? 0u : s + n;
+    return r;
+}
+
 // Group slot in exact mode: the tag is the key, so find-or-claim is one CAS
 // per probed slot with no lock.  Starts at slot s with n slots tried.
 // Returns -1 when the table is full.

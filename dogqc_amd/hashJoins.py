@@ -124,7 +124,8 @@ class EquiJoinTranslator(BinaryTranslator):
         self.storedBuffers = []
         if kind == "unique":
             prog.hostVars["mment%d" % idx] = "L.mment[%d] = 0;" % idx
-            tags = prog.buffer("jht%d_tags" % J, CType.UINT64, "L.htcap[%d]" % idx, "0x0ull")
+            # padded by PROBE_WINDOW - 1 slots for the probe's window loads
+            tags = prog.buffer("jht%d_tags" % J, CType.UINT64, "L.htcap[%d] + 3" % idx, "0x0ull")
             ftags = pipe.param("jht%d" % J, "uint64_t*", "B_" + tags)
             fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
             count = "L.htcap[%d]" % idx
@@ -323,15 +324,16 @@ class EquiJoinTranslator(BinaryTranslator):
             needCheck = (not self.exact) or a.conditions is not None
             jtag = pipe.reg("jtag%d" % J, "uint64_t")
             jc = pipe.reg("jc%d" % J, "uint32_t")
-            jt = pipe.reg("jt%d" % J, "uint64_t")
+            jw = pipe.reg("jw%d" % J, "dqc::probe_win")
             if not needCheck:
-                # first slot of every row of the thread first (independent loads)
+                # the probe window of every row of the thread first
+                # (independent loads), then the comparisons
                 with pipe.rowLoop():
                     pipe.emit("%s = -1;" % slot)
                     pipe.code.open("if (active[u])")
                     self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True)
                     pipe.emit("%s = dqc::probe_start(jh%d, %s);" % (jc, J, fcap))
-                    pipe.emit("%s = %s[%s];" % (jt, ftags, jc))
+                    pipe.emit("%s = dqc::probe_window(%s, %s);" % (jw, ftags, jc))
                     pipe.code.close()
             with pipe.rowLoop():
                 if needCheck:
@@ -346,13 +348,9 @@ class EquiJoinTranslator(BinaryTranslator):
                     pipe.emit("if (%s) break;" % self._matchCond(ctxt, pipe, fields, slotName))
                     pipe.code.close()
                 else:
-                    pipe.code.open("if (%s == %s)" % (jt, jtag))
-                    pipe.emit("%s = (int64_t)%s;" % (slot, jc))
-                    pipe.code.close()
-                    pipe.code.open("else if (%s != 0ull)" % jt)
-                    pipe.emit("uint32_t jcur = dqc::next_slot(%s, %s), jv = 1;" % (jc, fcap))
-                    pipe.emit("%s = dqc::ht_probe(%s, %s, %s, jcur, jv);" % (slot, ftags, fcap, jtag))
-                    pipe.code.close()
+                    pipe.emit("uint32_t jcur = 0, jv = 0;")
+                    pipe.emit("%s = dqc::probe_in_window(%s, %s, %s, %s, jcur, jv);" % (slot, jw, jc, fcap, jtag))
+                    pipe.emit("if (%s == -2) %s = dqc::ht_probe(%s, %s, %s, jcur, jv);" % (slot, slot, ftags, fcap, jtag))
                 if a.joinType == Join.ANTI:
                     pipe.emit("active[u] = %s < 0;" % slot)
                 else:

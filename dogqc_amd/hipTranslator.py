@@ -731,7 +731,8 @@ def pipelineShape(translator):
     """Walk up from a pipeline's source translator to its sink and report
     (hasInnerLoop, passFraction): whether an operator of the pipeline opens a
     per-lane loop (multimatch probe, lane refill) and the product of the
-    configured selectivities of the selections before the first join."""
+    configured selectivities of the selections before the first probe and of
+    that probe when it is a unique (single-match) join."""
     from dogqc_amd.hashJoins import EquiJoinTranslator
     from dogqc_amd.hipDivergenceBuffer import LaneRefill
     from dogqc_amd.relationalAlgebra import Join
@@ -744,6 +745,8 @@ def pipelineShape(translator):
         elif isinstance(p, EquiJoinTranslator):
             if p.leftChild is t:
                 break
+            if beforeJoin and not p.algExpr.multimatch and p.algExpr.joinType in (Join.INNER, Join.SEMI):
+                frac *= min(1.0, float(getattr(p.algExpr, "selectivity", 1.0)))
             beforeJoin = False
             if p.algExpr.multimatch and p.algExpr.joinType in (Join.INNER, Join.OUTER):
                 loops = True
@@ -1315,6 +1318,26 @@ class AggregationTranslator(UnaryTranslator):
         pro.emit("__syncthreads();")
         ep = pipe.epilogue
         ep.emit("__syncthreads();")
+        # few replicas: one thread per slot folds them; many: one wavefront
+        # per slot reduces them across lanes
+        ep.open("if (p.lr < 16u)")
+        ep.open("for (uint32_t s = threadIdx.x; s < p.lw; s += %du)" % pipe.bs)
+        ep.emit("const uint64_t tg = ltags[s];")
+        ep.emit("if (tg == 0) continue;")
+        for acc in self.accs:
+            ep.emit("%s f%d = lacc%d[s * p.lr];" % (acc["ctype"], acc["i"], acc["i"]))
+            ep.emit("for (uint32_t r = 1; r < p.lr; ++r) { const %s v = lacc%d[s * p.lr + r]; %s }"
+                    % (acc["ctype"], acc["i"], _combine(acc["kind"], "f%d" % acc["i"], "v")))
+        ep.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, tg, dqc::mix64(tg));" % (ftags, fcap))
+        ep.open("if (gs >= 0)")
+        for acc in self.accs:
+            ep.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], "f%d" % acc["i"]))
+        ep.close()
+        ep.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
+                % (1 << self.ht["index"]))
+        ep.close()
+        ep.close()
+        ep.open("else")
         ep.open("for (uint32_t s = wave; s < p.lw; s += %du)" % (pipe.bs // WAVE))
         ep.emit("const uint64_t tg = ltags[s];")
         ep.emit("if (tg == 0) continue;")
@@ -1331,6 +1354,7 @@ class AggregationTranslator(UnaryTranslator):
         ep.close()
         ep.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
                 % (1 << self.ht["index"]))
+        ep.close()
         ep.close()
         ep.close()
 

@@ -210,6 +210,7 @@ PLANS["c2_refill"] = ("c2", {"buffers": [3]})
 PLANS["c2_pushdown"] = ("c2", {"pushdown": True})
 PLANS["c2_both"] = ("c2", {"buffers": [3], "pushdown": True})
 PLANS["c3"] = ("c3", {})
+PLANS["c3_refill"] = ("c3", {"buffers": [6]})
 PLANS["cal"] = ("cal", {})
 for _n in FEATURE_PLANS:
     PLANS[_n] = ("feature", {})
@@ -234,7 +235,7 @@ def emptyAccessor(name):
 
 
 # plans the autotuner explores compile-time variants of (built ahead of time)
-TUNE_PLANS = ["c0", "c1", "c2_none", "c2_refill", "c2_pushdown", "c2_both"]
+TUNE_PLANS = ["c0", "c1", "c2_none", "c2_refill", "c2_pushdown", "c2_both", "c3", "c3_refill"]
 
 
 def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=None):

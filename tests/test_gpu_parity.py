@@ -157,8 +157,8 @@ def test_c2_full_closed_form(variant):
 C3_SMALL = {"sales": 200000, "item": 20000, "cust": 2000, "nation": 25}
 
 
-def c3Run(acc, tuning=None, storeKey=None):
-    mod = module("c3")
+def c3Run(acc, tuning=None, storeKey=None, plan="c3"):
+    mod = module(plan)
     store = _STORE.setdefault(storeKey, {}) if storeKey else {}
     inputs = rt.BoundInputs(mod, acc, store=store)
     ex = rt.run(mod, inputs, tuning or rt.newTuning())
@@ -178,14 +178,15 @@ def c3Oracle(acc):
     return res, ev.temps["c3_by_nation_year"][0]
 
 
+@pytest.mark.parametrize("plan", ["c3", "c3_refill"])
 @pytest.mark.parametrize("rows", [C3_SMALL, {"sales": 1, "item": 1, "cust": 1, "nation": 1},
                                   {"sales": 0, "item": 100, "cust": 10, "nation": 25},
                                   {"sales": 5000, "item": 0, "cust": 10, "nation": 25},
                                   {"sales": 70001, "item": 3000, "cust": 300, "nation": 5}])
-def test_c3_small(rows):
-    print("gpu case: c3 %r" % rows, flush=True)
+def test_c3_small(rows, plan):
+    print("gpu case: %s %r" % (plan, rows), flush=True)
     acc = workloadAcc("c3", rows, seed=41)
-    got, ny, _ = c3Run(acc)
+    got, ny, _ = c3Run(acc, plan=plan)
     want, wantNy = c3Oracle(acc)
     assertSameResult(got, want, {"revenue"}, label="c3 by item")
     assertSameResult(ny, wantNy, {"revenue"}, label="c3 by nation, year")
