@@ -1,6 +1,7 @@
 """Ahead-of-time build: the runtime library and every named plan module.
 
-``buildAll()`` compiles ``csrc/dqc_runtime.hip`` into ``libdqcrt.so`` and
+``buildAll()`` compiles ``csrc/dqc_runtime.hip`` and ``csrc/dqc_serve.hip``
+into ``libdqcrt.so`` and
 every plan of ``planlib.PLANS`` into ``_plans/`` (in parallel), always from
 source, for gfx950.
 """
@@ -13,18 +14,34 @@ from concurrent.futures import ThreadPoolExecutor
 from dogqc_amd import plancache
 
 PKG = plancache.PKG
-RUNTIME_SRC = os.path.join(PKG, "csrc", "dqc_runtime.hip")
+RUNTIME_SRCS = [os.path.join(PKG, "csrc", f) for f in ("dqc_runtime.hip", "dqc_serve.hip")]
 RUNTIME_SO = os.path.join(PKG, "libdqcrt.so")
 
 
 def buildRuntime():
     tmp = RUNTIME_SO + ".tmp"
-    cmd = [plancache.HIPCC] + plancache.HIPCC_FLAGS + ["-I", plancache.INCLUDE, RUNTIME_SRC, "-o", tmp, "-ldl"]
+    cmd = [plancache.HIPCC] + plancache.HIPCC_FLAGS + ["-I", plancache.INCLUDE] + RUNTIME_SRCS + \
+        ["-o", tmp, "-ldl", "-lpthread"]
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     if proc.returncode != 0:
         raise RuntimeError("building libdqcrt.so failed:\n" + proc.stderr[-6000:])
     os.replace(tmp, RUNTIME_SO)
     return RUNTIME_SO
+
+
+GEN_SRC = os.path.join(PKG, "csrc", "dqc_gen.hip")
+GEN_SO = os.path.join(PKG, "libdqcgen.so")
+
+
+def buildGen():
+    """libdqcgen.so: device-side generation of the benchmark tables."""
+    tmp = GEN_SO + ".tmp"
+    cmd = [plancache.HIPCC] + plancache.HIPCC_FLAGS + ["-I", plancache.INCLUDE, GEN_SRC, "-o", tmp]
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("building libdqcgen.so failed:\n" + proc.stderr[-6000:])
+    os.replace(tmp, GEN_SO)
+    return GEN_SO
 
 
 def buildPlans(names=None, jobs=None, force=False):
@@ -69,10 +86,12 @@ def cleanStale(keep):
 def buildAll(verbose=True):
     t0 = time.time()
     buildRuntime()
+    buildGen()
     paths = buildPlans(force=True)
     variants = buildVariants()
     cleanStale(list(paths.values()) + variants)
     if verbose:
-        print("built libdqcrt.so and %d plan modules in %.1f s" % (len(paths) + len(variants), time.time() - t0),
+        print("built libdqcrt.so, libdqcgen.so and %d plan modules in %.1f s" % (len(paths) + len(variants),
+                                                                                 time.time() - t0),
               file=sys.stderr)
     return paths

@@ -21,15 +21,22 @@
 #include <string>
 
 #include "dogqc_amd.h"
+#include "dqc_runtime_internal.h"
 
 static_assert(sizeof(dqc_plan_tuning) == 1256, "dqc_plan_tuning layout");
 static_assert(sizeof(dqc_plan_status) == 344, "dqc_plan_status layout");
 static_assert(sizeof(dqc_plan_layout) == 1648, "dqc_plan_layout layout");
 static_assert(sizeof(dqc_plan_inputs) == 24, "dqc_plan_inputs layout");
 
-namespace {
+using dqcrt::Module;
+using dqcrt::fail;
+using dqcrt::hipFail;
 
+namespace {
 thread_local std::string g_err;
+}  // namespace
+
+namespace dqcrt {
 
 int fail(int code, const char* fmt, ...) {
     char buf[1024];
@@ -45,20 +52,33 @@ int hipFail(hipError_t e, const char* what) {
     return fail(DQC_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define HIP_CHECK(expr, what)                      \
-    do {                                           \
-        hipError_t _e = (expr);                    \
-        if (_e != hipSuccess) return hipFail(_e, what); \
-    } while (0)
+bool adjustForRetry(const dqc_plan_layout& L, const dqc_plan_status& st, dqc_plan_tuning* tun) {
+    bool retry = false;
+    if (st.flags & DQC_ST_HT_OVERFLOW) {
+        for (int i = 0; i < L.nhts; ++i)
+            if (st.ht_overflow_mask & (1u << i)) tun->ht_capacity[i] = std::max<int64_t>(64, L.ht_capacity[i] * 2);
+        retry = true;
+    }
+    if (st.flags & DQC_ST_MM_OVERFLOW) {
+        for (int i = 0; i < L.nhts; ++i)
+            if (st.mm_overflow_mask & (1u << i))
+                tun->mm_entries[i] = std::max<int64_t>((int64_t)st.mm_total[i], L.mm_entries[i] + 1);
+        retry = true;
+    }
+    if (st.result_count > (uint64_t)L.result_capacity && L.nresult > 0) {
+        tun->result_capacity = (int64_t)st.result_count;
+        retry = true;
+    }
+    for (int i = 0; i < L.ntemps && i < DQC_MAX_TEMPS; ++i) {
+        if (st.temp_count[i] > (uint64_t)L.temp_capacity[i]) {
+            tun->temp_capacity[i] = (int64_t)st.temp_count[i];
+            retry = true;
+        }
+    }
+    return retry;
+}
 
-struct Module {
-    void* handle;
-    const char* (*describe)(void);
-    int (*layout)(const dqc_plan_inputs*, const dqc_plan_tuning*, dqc_plan_layout*);
-    int (*enqueue)(const dqc_plan_inputs*, const dqc_plan_tuning*, void*, uint64_t, void*, void* const*);
-};
-
-}  // namespace
+}  // namespace dqcrt
 
 extern "C" {
 
@@ -248,28 +268,7 @@ int dqc_query_run(void* module, const dqc_plan_inputs* in, dqc_plan_tuning* tun,
         res->layout = L;
         res->retries = attempt;
         const dqc_plan_status& st = res->status;
-        bool retry = false;
-        if (st.flags & DQC_ST_HT_OVERFLOW) {
-            for (int i = 0; i < L.nhts; ++i)
-                if (st.ht_overflow_mask & (1u << i)) tun->ht_capacity[i] = std::max<int64_t>(64, L.ht_capacity[i] * 2);
-            retry = true;
-        }
-        if (st.flags & DQC_ST_MM_OVERFLOW) {
-            for (int i = 0; i < L.nhts; ++i)
-                if (st.mm_overflow_mask & (1u << i))
-                    tun->mm_entries[i] = std::max<int64_t>((int64_t)st.mm_total[i], L.mm_entries[i] + 1);
-            retry = true;
-        }
-        if (st.result_count > (uint64_t)L.result_capacity && L.nresult > 0) {
-            tun->result_capacity = (int64_t)st.result_count;
-            retry = true;
-        }
-        for (int i = 0; i < L.ntemps && i < DQC_MAX_TEMPS; ++i) {
-            if (st.temp_count[i] > (uint64_t)L.temp_capacity[i]) {
-                tun->temp_capacity[i] = (int64_t)st.temp_count[i];
-                retry = true;
-            }
-        }
+        const bool retry = dqcrt::adjustForRetry(L, st, tun);
         if (!retry) return DQC_OK;
         if (attempt >= max_retries)
             return fail(DQC_EOVERFLOW, "tables still too small after %d retries (flags 0x%x)", attempt, st.flags);

@@ -1,0 +1,415 @@
+// Serving loop of libdqcrt.so (dqc_serve_* in include/dogqc_amd.h):
+// continuous batching of plan runs on one GPU.
+//
+// One scheduler thread owns every HIP call of a server.  Each loop turn it
+//   1. polls the completion event of every busy slot; a finished run's
+//      status (copied to pinned host memory on the slot's stream) decides
+//      between done and re-admission with enlarged tables,
+//   2. admits arrived queries in arrival order while a slot is free and the
+//      query's workspace fits the arena (a first-fit allocator over one
+//      device allocation made at creation, so admission never calls
+//      hipMalloc); a query waiting for memory may be passed by later, smaller
+//      ones for at most bypass_ms,
+//   3. enqueues an admitted plan, its status readback and an event on the
+//      slot's stream.
+// The client thread only submits requests and reads records.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "dogqc_amd.h"
+#include "dqc_runtime_internal.h"
+
+static_assert(sizeof(dqc_serve_config) == 48, "dqc_serve_config layout");
+static_assert(sizeof(dqc_serve_request) == 56, "dqc_serve_request layout");
+static_assert(sizeof(dqc_serve_record) == 56 + sizeof(dqc_plan_layout) + sizeof(dqc_plan_status),
+              "dqc_serve_record layout");
+
+namespace {
+
+using dqcrt::fail;
+using dqcrt::Module;
+
+constexpr uint64_t ARENA_ALIGN = 256;
+
+// First-fit allocator over [0, size) with coalescing of neighbours on free.
+class Arena {
+  public:
+    explicit Arena(uint64_t size) { free_[0] = size; }
+
+    bool alloc(uint64_t bytes, uint64_t* off) {
+        bytes = (bytes + ARENA_ALIGN - 1) & ~(ARENA_ALIGN - 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const uint64_t rest = it->second - bytes;
+                const uint64_t next = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[next] = rest;
+                used_[*off] = bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+
+    void release(uint64_t off) {
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        uint64_t start = off, len = u->second;
+        used_.erase(u);
+        auto nx = free_.lower_bound(start);
+        if (nx != free_.end() && start + len == nx->first) {
+            len += nx->second;
+            nx = free_.erase(nx);
+        }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == start) {
+                start = pv->first;
+                len += pv->second;
+                free_.erase(pv);
+            }
+        }
+        free_[start] = len;
+    }
+
+  private:
+    std::map<uint64_t, uint64_t> free_;   // offset -> length
+    std::map<uint64_t, uint64_t> used_;   // offset -> length
+};
+
+struct Request {
+    int64_t id = 0;
+    Module* module = nullptr;
+    std::vector<int64_t> rows;
+    std::vector<const void*> cols;
+    dqc_plan_inputs inputs{};
+    dqc_plan_tuning tuning{};
+    dqc_serve_record rec{};
+    uint64_t arenaOff = 0;
+    bool hasWs = false;
+    int64_t blockedSince = -1;   // first time it waited for memory at the head of the queue
+    bool keep = false;
+
+    void bindInputs() {
+        inputs.table_rows = rows.data();
+        inputs.cols = cols.data();
+    }
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int64_t req = -1;
+};
+
+struct Server {
+    dqc_serve_config cfg{};
+    int device = 0;
+    void* arenaBase = nullptr;
+    std::unique_ptr<Arena> arena;
+    std::vector<Slot> slots;
+    dqc_plan_status* pinned = nullptr;   // one status per slot
+
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::unique_ptr<Request>> reqs;
+    std::deque<int64_t> pending;          // not yet admitted, in submit order
+    int64_t unfinished = 0;
+    std::vector<int64_t> toRelease;       // kept workspaces the client released
+
+    std::thread worker;
+    std::atomic<bool> started{false};
+    std::atomic<bool> stop{false};
+    std::chrono::steady_clock::time_point t0;
+    int fatal = 0;
+
+    int64_t now() const {
+        return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void finish(Request& r, int state, int error) {
+        r.rec.state = state;
+        r.rec.error = error;
+        r.rec.done_ns = now();
+        --unfinished;
+    }
+
+    void freeWs(Request& r) {
+        if (r.hasWs) arena->release(r.arenaOff);
+        r.hasWs = false;
+        r.rec.ws = nullptr;
+    }
+
+    // Admission of one request on slot `si`; false when it does not fit now.
+    bool tryAdmit(int64_t id, int si, int64_t t) {
+        Request& r = *reqs[id];
+        dqc_plan_layout L;
+        int rc = r.module->layout(&r.inputs, &r.tuning, &L);
+        if (rc) {
+            fail(rc, "plan layout failed (%d)", rc);
+            finish(r, DQC_SERVE_FAILED, rc);
+            return true;
+        }
+        if (L.ws_bytes > cfg.arena_bytes) {
+            finish(r, DQC_SERVE_FAILED, DQC_ENOMEM);
+            return true;
+        }
+        uint64_t off = 0;
+        if (!arena->alloc(L.ws_bytes, &off)) return false;
+        r.arenaOff = off;
+        r.hasWs = true;
+        r.rec.ws = (char*)arenaBase + off;
+        r.rec.ws_bytes = L.ws_bytes;
+        r.rec.layout = L;
+        r.rec.admit_ns = t;
+        r.rec.slot = si;
+        r.rec.state = DQC_SERVE_RUNNING;
+        Slot& s = slots[si];
+        rc = r.module->enqueue(&r.inputs, &r.tuning, r.rec.ws, L.ws_bytes, (void*)s.stream, nullptr);
+        hipError_t e = hipSuccess;
+        if (rc == DQC_OK) e = hipMemcpyAsync(&pinned[si], r.rec.ws, sizeof(dqc_plan_status), hipMemcpyDeviceToHost,
+                                            s.stream);
+        if (rc == DQC_OK && e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+        if (rc != DQC_OK || e != hipSuccess) {
+            // nothing of this query can be pending on a stream it failed to use
+            (void)hipStreamSynchronize(s.stream);
+            freeWs(r);
+            finish(r, DQC_SERVE_FAILED, rc != DQC_OK ? rc : DQC_EHIP);
+            return true;
+        }
+        s.req = id;
+        return true;
+    }
+
+    // One scheduler turn; returns whether anything happened.
+    bool turn() {
+        bool progress = false;
+        std::unique_lock<std::mutex> lk(mu);
+        for (int64_t id : toRelease) freeWs(*reqs[id]);
+        toRelease.clear();
+        // 1. completions
+        for (size_t si = 0; si < slots.size(); ++si) {
+            Slot& s = slots[si];
+            if (s.req < 0) continue;
+            hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            Request& r = *reqs[s.req];
+            s.req = -1;
+            progress = true;
+            if (q != hipSuccess) {
+                freeWs(r);
+                finish(r, DQC_SERVE_FAILED, DQC_EHIP);
+                fatal = DQC_EHIP;
+                continue;
+            }
+            r.rec.status = pinned[si];
+            if (dqcrt::adjustForRetry(r.rec.layout, r.rec.status, &r.tuning)) {
+                freeWs(r);
+                if (r.rec.retries >= cfg.max_retries) {
+                    finish(r, DQC_SERVE_FAILED, DQC_EOVERFLOW);
+                } else {
+                    ++r.rec.retries;
+                    r.rec.state = DQC_SERVE_PENDING;
+                    enqueuePending(r.id);
+                }
+                continue;
+            }
+            if (!r.keep) freeWs(r);
+            finish(r, DQC_SERVE_DONE, 0);
+        }
+        // 2. admissions in arrival order (stamped after the completions of
+        // this turn, whose slots and memory they may take over)
+        const int64_t ta = now();
+        for (auto it = pending.begin(); it != pending.end();) {
+            int si = freeSlot();
+            if (si < 0) break;
+            Request& r = *reqs[*it];
+            if (r.rec.arrival_ns > ta) break;   // pending is kept in arrival order
+            if (tryAdmit(*it, si, ta)) {
+                progress = true;
+                it = pending.erase(it);
+                continue;
+            }
+            // does not fit: wait, or let later queries pass for a while
+            if (r.blockedSince < 0) r.blockedSince = ta;
+            if (cfg.bypass_ms <= 0 || ta - r.blockedSince > (int64_t)cfg.bypass_ms * 1000000) break;
+            ++it;
+        }
+        if (progress) cv.notify_all();
+        return progress;
+    }
+
+    // insert into `pending`, which is kept in arrival order
+    void enqueuePending(int64_t id) {
+        const int64_t a = reqs[id]->rec.arrival_ns;
+        auto pos = pending.end();
+        while (pos != pending.begin() && reqs[*std::prev(pos)]->rec.arrival_ns > a) --pos;
+        pending.insert(pos, id);
+    }
+
+    int freeSlot() const {
+        for (size_t i = 0; i < slots.size(); ++i)
+            if (slots[i].req < 0) return (int)i;
+        return -1;
+    }
+
+    bool busy() const {
+        for (const Slot& s : slots)
+            if (s.req >= 0) return true;
+        return false;
+    }
+
+    void run() {
+        (void)hipSetDevice(device);
+        while (true) {
+            const bool progress = turn();
+            if (stop.load()) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!busy()) break;
+            }
+            if (!progress) std::this_thread::yield();
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int dqc_serve_create(const dqc_serve_config* cfg, void** server) {
+    if (!cfg || !server) return fail(DQC_EINVAL, "dqc_serve_create: null argument");
+    if (cfg->slots < 1 || cfg->slots > 64) return fail(DQC_EINVAL, "slots must be in 1..64, got %d", cfg->slots);
+    if (cfg->arena_bytes == 0) return fail(DQC_EINVAL, "arena_bytes must be positive");
+    std::unique_ptr<Server> s(new Server());
+    s->cfg = *cfg;
+    HIP_CHECK(hipGetDevice(&s->device), "hipGetDevice");
+    void* base = nullptr;
+    hipError_t e = hipMalloc(&base, cfg->arena_bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(DQC_ENOMEM, "arena of %llu bytes: %s", (unsigned long long)cfg->arena_bytes,
+                    hipGetErrorString(e));
+    }
+    s->arenaBase = base;
+    s->arena.reset(new Arena(cfg->arena_bytes));
+    e = hipHostMalloc((void**)&s->pinned, sizeof(dqc_plan_status) * cfg->slots, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipFree(base);
+        return dqcrt::hipFail(e, "hipHostMalloc");
+    }
+    s->slots.resize(cfg->slots);
+    for (Slot& sl : s->slots) {
+        HIP_CHECK(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "hipStreamCreate");
+        HIP_CHECK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate");
+    }
+    *server = s.release();
+    return DQC_OK;
+}
+
+int dqc_serve_submit(void* server, const dqc_serve_request* req, int64_t* id) {
+    Server* s = (Server*)server;
+    if (!s || !req || !req->module) return fail(DQC_EINVAL, "dqc_serve_submit: null argument");
+    if (req->inputs.ntables < 0 || req->inputs.ncols < 0) return fail(DQC_EINVAL, "bad input binding");
+    std::unique_ptr<Request> r(new Request());
+    r->module = (Module*)req->module;
+    r->rows.assign(req->inputs.table_rows, req->inputs.table_rows + req->inputs.ntables);
+    r->cols.assign(req->inputs.cols, req->inputs.cols + req->inputs.ncols);
+    r->inputs.ntables = req->inputs.ntables;
+    r->inputs.ncols = req->inputs.ncols;
+    r->bindInputs();
+    if (req->tuning) r->tuning = *req->tuning;
+    r->keep = req->keep_result != 0;
+    r->rec.arrival_ns = req->arrival_ns;
+    r->rec.state = DQC_SERVE_PENDING;
+    r->rec.slot = -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int64_t n = (int64_t)s->reqs.size();
+    r->id = n;
+    s->reqs.push_back(std::move(r));
+    s->enqueuePending(n);
+    ++s->unfinished;
+    if (id) *id = n;
+    return DQC_OK;
+}
+
+int dqc_serve_start(void* server) {
+    Server* s = (Server*)server;
+    if (!s) return fail(DQC_EINVAL, "null server");
+    if (s->started.exchange(true)) return fail(DQC_EINVAL, "server already started");
+    s->t0 = std::chrono::steady_clock::now();
+    s->worker = std::thread([s] { s->run(); });
+    return DQC_OK;
+}
+
+int dqc_serve_wait(void* server, int64_t timeout_ms) {
+    Server* s = (Server*)server;
+    if (!s) return fail(DQC_EINVAL, "null server");
+    if (!s->started.load()) return fail(DQC_EINVAL, "server not started");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto pred = [s] { return s->unfinished == 0 || s->fatal != 0; };
+    if (timeout_ms < 0) {
+        s->cv.wait(lk, pred);
+    } else if (!s->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred)) {
+        return fail(DQC_EINVAL, "dqc_serve_wait: %lld queries unfinished after %lld ms", (long long)s->unfinished,
+                    (long long)timeout_ms);
+    }
+    if (s->fatal) return fail(s->fatal, "a query failed on the device");
+    return DQC_OK;
+}
+
+int dqc_serve_record_of(void* server, int64_t id, dqc_serve_record* out) {
+    Server* s = (Server*)server;
+    if (!s || !out) return fail(DQC_EINVAL, "dqc_serve_record_of: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int64_t)s->reqs.size()) return fail(DQC_EINVAL, "no query %lld", (long long)id);
+    *out = s->reqs[id]->rec;
+    return DQC_OK;
+}
+
+int dqc_serve_release(void* server, int64_t id) {
+    Server* s = (Server*)server;
+    if (!s) return fail(DQC_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < 0 || id >= (int64_t)s->reqs.size()) return fail(DQC_EINVAL, "no query %lld", (long long)id);
+    Request& r = *s->reqs[id];
+    if (r.rec.state != DQC_SERVE_DONE) return fail(DQC_EINVAL, "query %lld is not done", (long long)id);
+    if (!r.hasWs) return DQC_OK;
+    if (s->started.load() && !s->stop.load())
+        s->toRelease.push_back(id);   // the scheduler thread owns the arena
+    else
+        s->freeWs(r);
+    r.rec.ws = nullptr;
+    return DQC_OK;
+}
+
+int dqc_serve_destroy(void* server) {
+    Server* s = (Server*)server;
+    if (!s) return DQC_OK;
+    if (s->started.load()) {
+        s->stop.store(true);
+        s->worker.join();
+    }
+    for (Slot& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    if (s->pinned) (void)hipHostFree(s->pinned);
+    if (s->arenaBase) (void)hipFree(s->arenaBase);
+    delete s;
+    return DQC_OK;
+}
+
+}  // extern "C"

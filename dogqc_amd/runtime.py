@@ -23,7 +23,11 @@ RUNTIME_SYMBOLS = [
     "dqc_event_destroy", "dqc_event_elapsed_ms", "dqc_event_record", "dqc_mmdb_load", "dqc_module_load",
     "dqc_module_unload", "dqc_module_describe", "dqc_query_layout", "dqc_query_enqueue", "dqc_query_run",
     "dqc_query_release", "dqc_graph_create", "dqc_graph_launch", "dqc_graph_destroy",
+    "dqc_serve_create", "dqc_serve_submit", "dqc_serve_start", "dqc_serve_wait", "dqc_serve_record_of",
+    "dqc_serve_release", "dqc_serve_destroy",
 ]
+GEN_SYMBOLS = ["dqc_gen_uniform", "dqc_gen_decimal", "dqc_gen_pick_u32", "dqc_gen_affine_perm",
+               "dqc_gen_inverse_cdf", "dqc_gen_text"]
 PLAN_SYMBOLS = ["dqc_plan_describe", "dqc_plan_layout_of", "dqc_plan_enqueue"]
 
 
@@ -58,6 +62,22 @@ class Layout(C.Structure):
 class QueryResult(C.Structure):
     _fields_ = [("ws", C.c_void_p), ("ws_bytes", C.c_uint64), ("layout", Layout), ("status", Status),
                 ("retries", C.c_int32)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_retries", C.c_int32), ("arena_bytes", C.c_uint64),
+                ("bypass_ms", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("module", C.c_void_p), ("inputs", Inputs), ("tuning", C.POINTER(Tuning)),
+                ("arrival_ns", C.c_int64), ("keep_result", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServeRecord(C.Structure):
+    _fields_ = [("arrival_ns", C.c_int64), ("admit_ns", C.c_int64), ("done_ns", C.c_int64),
+                ("state", C.c_int32), ("retries", C.c_int32), ("error", C.c_int32), ("slot", C.c_int32),
+                ("ws", C.c_void_p), ("ws_bytes", C.c_uint64), ("layout", Layout), ("status", Status)]
 
 
 class DqcError(RuntimeError):
@@ -102,6 +122,13 @@ def lib():
         L.dqc_graph_create.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), vp, C.c_uint64, vp, C.POINTER(vp)]
         L.dqc_graph_launch.argtypes = [vp, vp]
         L.dqc_graph_destroy.argtypes = [vp]
+        L.dqc_serve_create.argtypes = [C.POINTER(ServeConfig), C.POINTER(vp)]
+        L.dqc_serve_submit.argtypes = [vp, C.POINTER(ServeRequest), C.POINTER(C.c_int64)]
+        L.dqc_serve_start.argtypes = [vp]
+        L.dqc_serve_wait.argtypes = [vp, C.c_int64]
+        L.dqc_serve_record_of.argtypes = [vp, C.c_int64, C.POINTER(ServeRecord)]
+        L.dqc_serve_release.argtypes = [vp, C.c_int64]
+        L.dqc_serve_destroy.argtypes = [vp]
         _lib = L
     return _lib
 

@@ -1,0 +1,319 @@
+"""Serving workload: a stream of queries over HBM-resident tables, run
+through the native serving loop of libdqcrt.so (dqc_serve_*).
+
+Each query is one of the four benchmark plans (c0 scan-aggregate, c1 PK-FK
+join, c2 expanding join, c3 star join) over a row prefix of the resident
+tables: a row scale s, log-uniform in [0.1, 10], selects the first
+s x (base rows) rows of every scaled table, the row-range predicate of the
+workload.  The loop admits arrived queries while a slot (HIP stream) is
+free and the query's workspace fits the arena, so queries of uneven sizes
+overlap on the device; with one slot it is the one-query-at-a-time
+baseline on the same code path.
+
+``benchmark()`` measures both: capacity (all queries arrive at once,
+queries per second over the makespan) and latency under a Poisson stream at
+a fixed rate (p50/p99 of arrival-to-completion), and checks a sample of the
+results against the CPU oracle and between the alone and mixed runs.
+"""
+import ctypes as C
+import math
+import statistics
+import time
+from collections import OrderedDict
+
+import numpy as np
+
+from dogqc_amd import planlib, runtime as rt, runner, workloads
+
+CONFIG_PLANS = OrderedDict([("c0", "c0"), ("c1", "c1"), ("c2", "c2_pushdown"), ("c3", "c3_refill")])
+SCALE_RANGE = (0.1, 10.0)
+
+
+class Server(object):
+    """ctypes wrapper of one dqc_serve server."""
+
+    def __init__(self, slots, arena_bytes, max_retries=8, bypass_ms=50):
+        cfg = rt.ServeConfig()
+        cfg.slots, cfg.max_retries, cfg.arena_bytes, cfg.bypass_ms = slots, max_retries, int(arena_bytes), bypass_ms
+        self.handle = C.c_void_p()
+        rt.check(rt.lib().dqc_serve_create(C.byref(cfg), C.byref(self.handle)), "dqc_serve_create")
+
+    def submit(self, module, rows, cols, tuning=None, arrival_ns=0, keep=False):
+        req = rt.ServeRequest()
+        req.module = module.handle
+        req.inputs = rt.Inputs(len(rows), len(cols), rows.ctypes.data_as(C.POINTER(C.c_int64)),
+                               C.cast(cols, C.POINTER(C.c_void_p)))
+        req.tuning = C.pointer(tuning) if tuning is not None else None
+        req.arrival_ns = int(arrival_ns)
+        req.keep_result = 1 if keep else 0
+        qid = C.c_int64()
+        rt.check(rt.lib().dqc_serve_submit(self.handle, C.byref(req), C.byref(qid)), "dqc_serve_submit")
+        return qid.value
+
+    def start(self):
+        rt.check(rt.lib().dqc_serve_start(self.handle), "dqc_serve_start")
+
+    def wait(self, timeout_ms=-1):
+        rt.check(rt.lib().dqc_serve_wait(self.handle, int(timeout_ms)), "dqc_serve_wait")
+
+    def record(self, qid):
+        rec = rt.ServeRecord()
+        rt.check(rt.lib().dqc_serve_record_of(self.handle, qid, C.byref(rec)), "dqc_serve_record_of")
+        return rec
+
+    def release(self, qid):
+        rt.check(rt.lib().dqc_serve_release(self.handle, qid), "dqc_serve_release")
+
+    def destroy(self):
+        if self.handle:
+            rt.check(rt.lib().dqc_serve_destroy(self.handle), "dqc_serve_destroy")
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class PlanSet(object):
+    """The plan module and runtime tuning of every config (the autotuned
+    variant when a record for the base size exists)."""
+
+    def __init__(self, configs=tuple(CONFIG_PLANS), tuned=True):
+        from dogqc_amd import autotune
+        self.modules, self.tunings, self.names = {}, {}, {}
+        for cfg in configs:
+            name = CONFIG_PLANS[cfg]
+            rec = None
+            if tuned:
+                acc = workloads.emptyAccessor(cfg)
+                for t, n in workloads.FULL_ROWS[cfg].items():
+                    acc.schema[t]["size"] = n
+                rec = autotune.loadCached(name, acc)
+            if rec is not None:
+                comp = autotune.translateVariant(name, rec["block"], rec["rows_per_thread"])
+                mod = rt.PlanModule(comp.compile(autotune.variantName(name, rec["block"], rec["rows_per_thread"])))
+                tun = autotune.tuningFromDict(rec["tuning"])
+            else:
+                mod = rt.PlanModule(planlib.translate(name).compile(name))
+                tun = rt.newTuning()
+            self.modules[cfg], self.tunings[cfg], self.names[cfg] = mod, tun, name
+
+
+class Query(object):
+    def __init__(self, config, scale, arrival_ns=0):
+        self.config, self.scale, self.arrival_ns = config, scale, arrival_ns
+
+
+def prefixRows(tables, config, scale):
+    """Rows of every table of a config a query at `scale` reads: the first
+    scale x base rows (the 25 nations are always read whole)."""
+    rows = OrderedDict()
+    for t, n in workloads.FULL_ROWS[config].items():
+        if config == "c3" and t == "nation":
+            rows[t] = tables.rows[t]
+        else:
+            rows[t] = max(1, min(tables.rows[t], int(round(n * scale))))
+    return rows
+
+
+def binding(module, tables, rows):
+    """(table_rows, column pointers) of a module over resident tables."""
+    desc = module.desc
+    r = np.array([rows[t] for t in desc["tables"]], dtype=np.int64)
+    cols = (C.c_void_p * max(1, len(desc["columns"])))()
+    for c in desc["columns"]:
+        cols[c["index"]] = tables.store[(c["table"], c["attr"], c["part"])].ptr
+    return r, cols
+
+
+def workspaceBytes(tables, plans, config, scale):
+    """Workspace bytes of one query (its admission footprint)."""
+    r, cols = binding(plans.modules[config], tables, prefixRows(tables, config, scale))
+    inputs = rt.Inputs(len(r), len(cols), r.ctypes.data_as(C.POINTER(C.c_int64)), C.cast(cols, C.POINTER(C.c_void_p)))
+    lay = rt.Layout()
+    rt.check(rt.lib().dqc_query_layout(plans.modules[config].handle, C.byref(inputs), C.byref(plans.tunings[config]),
+                                       C.byref(lay)), "dqc_query_layout")
+    return int(lay.ws_bytes)
+
+
+def makeMix(n, seed=4, configs=tuple(CONFIG_PLANS), scaleRange=SCALE_RANGE):
+    """n queries: config uniform, row scale log-uniform over scaleRange."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = math.log(scaleRange[0]), math.log(scaleRange[1])
+    cfg = rng.integers(0, len(configs), size=n)
+    sc = np.exp(rng.uniform(lo, hi, size=n))
+    return [Query(configs[c], float(s)) for c, s in zip(cfg, sc)]
+
+
+def poissonArrivals(queries, qps, seed=5):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gaps = rng.exponential(1.0 / qps, size=len(queries))
+    t = np.cumsum(gaps) - gaps[0]
+    for q, a in zip(queries, t):
+        q.arrival_ns = int(a * 1e9)
+
+
+def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600):
+    """Run queries through a server with `slots` slots; returns (records,
+    server) -- the server stays alive so kept results can be read."""
+    srv = Server(slots, arena_bytes)
+    ids = []
+    for i, q in enumerate(queries):
+        rows = prefixRows(tables, q.config, q.scale)
+        r, cols = binding(plans.modules[q.config], tables, rows)
+        ids.append(srv.submit(plans.modules[q.config], r, cols, plans.tunings[q.config], q.arrival_ns,
+                              keep=i in keep))
+    srv.start()
+    srv.wait(int(timeout_s * 1000))
+    recs = [srv.record(i) for i in ids]
+    for i, rec in enumerate(recs):
+        if rec.state != 2:
+            raise rt.DqcError("query %d (%s x%.2f) failed with code %d" % (i, queries[i].config, queries[i].scale,
+                                                                           rec.error))
+    return recs, srv
+
+
+def latencyStats(recs):
+    lat = np.array([(r.done_ns - r.arrival_ns) * 1e-6 for r in recs])
+    first = min(r.arrival_ns for r in recs)
+    last = max(r.done_ns for r in recs)
+    return {"queries": len(recs), "makespan_ms": (last - first) * 1e-6,
+            "qps": len(recs) / ((last - first) * 1e-9), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+            "retries": int(sum(r.retries for r in recs))}
+
+
+def resultOf(plans, config, rec):
+    """Result columns (and c3's second aggregation) of a kept record."""
+    res = rt.QueryResult()
+    res.ws, res.ws_bytes, res.layout, res.status = rec.ws, rec.ws_bytes, rec.layout, rec.status
+    mod = plans.modules[config]
+    ex = rt.Execution(mod, None, None, res)
+    ex.owned = False
+    out = runner.resultColumns(mod, ex, None)
+    if config == "c3":
+        out = (out, runner.tempColumns(mod, ex, None, "c3_by_nation_year"))
+    return out
+
+
+def sameResult(config, a, b, rtol=1e-9):
+    """Multiset equality of two results of one config: keys and counts
+    bit-exact, float64 sums within rtol."""
+    if config == "c3":
+        return sameResult("c3a", a[0], b[0], rtol) and sameResult("c3b", a[1], b[1], rtol)
+    keys = {"c0": ["g1", "g2"], "c1": ["d_key"], "c2": [], "c3a": ["i_key"], "c3b": ["n_key", "c_year"]}[config]
+    tol = {"c0": {"sum_base_price", "sum_disc_price", "sum_charge", "avg_price", "avg_disc", "avg_qty", "sum_qty"},
+           "c1": {"revenue"}, "c2": set(), "c3a": {"revenue"}, "c3b": {"revenue"}}[config]
+    if list(a) != list(b):
+        return False
+    n = len(next(iter(a.values())))
+    if any(len(v) != n for v in list(a.values()) + list(b.values())):
+        return False
+    oa = np.lexsort([np.asarray(a[k]) for k in reversed(keys)]) if keys else np.arange(n)
+    ob = np.lexsort([np.asarray(b[k]) for k in reversed(keys)]) if keys else np.arange(n)
+    for k in a:
+        x, y = np.asarray(a[k])[oa], np.asarray(b[k])[ob]
+        if k in tol:
+            if not np.allclose(x, y, rtol=rtol, atol=0):
+                return False
+        elif not np.array_equal(x, y):
+            return False
+    return True
+
+
+def oracleResult(tables, config, scale):
+    """The CPU oracle's result of one query (host copies of the prefixes)."""
+    from oracle import coracle
+    rows = prefixRows(tables, config, scale)
+    acc = tables.hostAccessor(rows)
+    cols = {t: {a: acc.column(t, a) for a in acc.schema[t]["attributes"]} for t in rows}
+    if config == "c0":
+        ref = coracle.c0(cols["cfg0"])
+        out = OrderedDict((k, []) for k in ("g1", "g2", "sum_qty", "sum_base_price", "sum_disc_price",
+                                             "sum_charge", "count_order"))
+        for (g1, g2), v in sorted(ref.items()):
+            for k, x in zip(out, (g1, g2) + tuple(v[:4]) + (v[5],)):
+                out[k].append(x)
+        return OrderedDict((k, np.array(v)) for k, v in out.items())
+    if config == "c1":
+        keys, sums = coracle.c1(cols["dim"], cols["fact"])
+        return OrderedDict([("d_key", keys), ("revenue", sums)])
+    if config == "c2":
+        cnt, s = coracle.c2(cols["build"], cols["probe"])
+        return OrderedDict([("cnt", np.array([cnt], dtype=np.int32)), ("sum_prod", np.array([s], dtype=np.int64))])
+    return coracle.c3(cols, workloads.C3_WORD)
+
+
+def matchesOracle(config, got, want):
+    if config == "c0":
+        sub = OrderedDict((k, np.asarray(got[k])) for k in want)
+        sub["count_order"] = sub["count_order"].astype(np.int64)
+        want = OrderedDict(want)
+        want["count_order"] = want["count_order"].astype(np.int64)
+        want["g1"], want["g2"] = want["g1"].astype(np.int8), want["g2"].astype(np.int8)
+        return sameResult("c0", sub, want)
+    if config == "c2":
+        return int(got["cnt"][0]) == int(want["cnt"][0]) and int(got["sum_prod"][0]) == int(want["sum_prod"][0])
+    return sameResult(config, got, want)
+
+
+def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None):
+    """Serving benchmark on resident tables at `scale` x the base sizes.
+    Returns a dict of capacity (serial / batched QPS), latency at a fixed
+    Poisson rate (load x the serial capacity) and verification results."""
+    from dogqc_amd import datagen
+    say = log or (lambda *a: None)
+    t0 = time.perf_counter()
+    tables = datagen.ResidentTables(scale=scale)
+    say("resident tables: %.1f GB in %.1f s" % (tables.nbytes() / 1e9, time.perf_counter() - t0))
+    plans = PlanSet()
+    arena = int(arena_gb * (1 << 30))
+    # warm up every module once (code objects load at their first launch)
+    run(tables, plans, [Query(c, 0.1) for c in CONFIG_PLANS], 1, arena)[1].destroy()
+    queries = makeMix(nqueries, seed)
+    # the verified sample: the smallest queries of each config, so the CPU
+    # oracle over host copies of their prefixes stays within seconds
+    order = sorted(range(nqueries), key=lambda i: queries[i].scale)
+    sample = []
+    for c in CONFIG_PLANS:
+        sample += [i for i in order if queries[i].config == c][:max(1, verify // len(CONFIG_PLANS))]
+    out = {"scale": scale, "resident_gb": tables.nbytes() / 1e9, "queries": nqueries, "slots": slots,
+           "arena_gb": arena_gb, "plans": dict(plans.names)}
+    # capacity: every query arrives at t = 0
+    for q in queries:
+        q.arrival_ns = 0
+    recsS, srvS = run(tables, plans, queries, 1, arena, keep=set(sample))
+    out["serial"] = latencyStats(recsS)
+    say("serial capacity: %.1f q/s" % out["serial"]["qps"])
+    recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample))
+    out["batched"] = latencyStats(recsB)
+    say("batched capacity: %.1f q/s" % out["batched"]["qps"])
+    # alone vs mixed, and the CPU oracle, on the sample
+    alone_same, oracle_ok = 0, 0
+    for i in sample:
+        q = queries[i]
+        a = resultOf(plans, q.config, recsS[i])
+        b = resultOf(plans, q.config, recsB[i])
+        alone_same += sameResult(q.config, a, b)
+        oracle_ok += matchesOracle(q.config, b, oracleResult(tables, q.config, q.scale))
+    out["verify"] = {"sample": len(sample), "alone_equals_mixed": alone_same, "oracle_ok": oracle_ok,
+                     "ok": alone_same == len(sample) and oracle_ok == len(sample)}
+    srvS.destroy()
+    srvB.destroy()
+    say("verified %d queries: alone == mixed %d, oracle %d" % (len(sample), alone_same, oracle_ok))
+    # latency at a fixed offered rate
+    rate = load * out["serial"]["qps"]
+    poissonArrivals(queries, rate, seed + 1)
+    recs, srv = run(tables, plans, queries, 1, arena)
+    srv.destroy()
+    out["serial_at_rate"] = latencyStats(recs)
+    recs, srv = run(tables, plans, queries, slots, arena)
+    srv.destroy()
+    out["batched_at_rate"] = latencyStats(recs)
+    out["offered_qps"] = rate
+    out["speedup"] = out["batched"]["qps"] / out["serial"]["qps"]
+    say("at %.1f q/s: p99 serial %.2f ms, batched %.2f ms" % (rate, out["serial_at_rate"]["p99_ms"],
+                                                            out["batched_at_rate"]["p99_ms"]))
+    return out

@@ -10,10 +10,11 @@ from dogqc_amd import runtime as rt, planlib, build
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "dogqc_amd.h")
+GEN_HEADER = os.path.join(ROOT, "include", "dogqc_amd_datagen.h")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(dqc_[a-z0-9_]+)\s*\(", text)))
 
@@ -35,6 +36,22 @@ def test_header_lists_runtime_and_plan_abi():
 def test_runtime_exports(runtimeLib):
     for s in rt.RUNTIME_SYMBOLS:
         assert hasattr(runtimeLib, s), s
+
+
+def test_datagen_header_and_exports():
+    from dogqc_amd import datagen
+    assert declared(GEN_HEADER) == sorted(rt.GEN_SYMBOLS)
+    if not os.path.exists(datagen.GEN_SO):
+        build.buildGen()
+    lib = ctypes.CDLL(datagen.GEN_SO)
+    for s in rt.GEN_SYMBOLS:
+        assert hasattr(lib, s), s
+
+
+def test_serve_structs_match_header():
+    assert ctypes.sizeof(rt.ServeConfig) == 4 + 4 + 8 + 4 + 4 * 7 == 48
+    assert ctypes.sizeof(rt.ServeRequest) == 8 + 24 + 8 + 8 + 4 + 4 == 56
+    assert ctypes.sizeof(rt.ServeRecord) == 8 * 3 + 4 * 4 + 8 + 8 + ctypes.sizeof(rt.Layout) + ctypes.sizeof(rt.Status)
 
 
 def test_plan_module_exports_and_describe():

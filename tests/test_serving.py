@@ -1,0 +1,157 @@
+"""Serving loop (dqc_serve_*) and device-side table generation, on the GPU.
+
+  * the generator kernels against a numpy restatement of their formulas;
+  * every query of a mixed stream gives the same result alone (one slot)
+    and in the mix (eight slots), and the CPU oracle's result;
+  * admission never runs more queries at once than their workspaces fit in
+    the arena, a query larger than the arena fails cleanly, and a query
+    whose tables overflow is re-admitted with larger ones.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dogqc_amd import datagen, runtime as rt, serving, workloads
+
+pytestmark = pytest.mark.gpu
+
+M64 = (1 << 64) - 1
+
+
+def rnd(seed, column, rows):
+    """The generator's random word of (seed, column, row) for a row array."""
+    rows = rows.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) ^ (np.uint64(column) * np.uint64(0x9E3779B97F4A7C15))
+        z = z + (rows + np.uint64(1)) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(r, lo, hi):
+    rng = np.uint64(hi - lo + 1)
+    return lo + (((r >> np.uint64(32)) * rng) >> np.uint64(32)).astype(np.int64)
+
+
+_TABLES = {}
+
+
+def tables(scale):
+    if scale not in _TABLES:
+        _TABLES[scale] = datagen.ResidentTables(scale=scale, seed=77)
+    return _TABLES[scale]
+
+
+def test_datagen_matches_restatement():
+    print("gpu case: datagen restatement", flush=True)
+    T = tables(0.02)
+    seed = T.seed
+    cid = T.columnIds
+    n = T.rows["cfg0"]
+    rows = np.arange(n)
+    assert cid[("cfg0", "g1")] == 16 and cid[("cfg0", "qty")] == 48
+    g1 = T.store[("cfg0", "g1", None)].toArray(np.int8, n)
+    assert np.array_equal(g1, uniform(rnd(seed, 16, rows), 0, 2).astype(np.int8))
+    qty = T.store[("cfg0", "qty", None)].toArray(np.int32, n)
+    assert np.array_equal(qty, uniform(rnd(seed, 48, rows), 1, 50).astype(np.int32))
+    price = T.store[("cfg0", "price", None)].toArray(np.float64, n)
+    assert np.array_equal(price, uniform(rnd(seed, cid[("cfg0", "price")], rows), 90000, 10500000)
+                          .astype(np.float64) / 100.0)
+    date = T.store[("cfg0", "date", None)].toArray(np.uint32, n)
+    assert np.array_equal(date, workloads.dateTable()[uniform(rnd(seed, cid[("cfg0", "date")], rows), 0, 2556)])
+    nf = T.rows["fact"]
+    fk = T.store[("fact", "f_fk", None)].toArray(np.int32, nf)
+    assert np.array_equal(fk, uniform(rnd(seed, cid[("fact", "f_fk")], np.arange(nf)), 0, T.rows["dim"] - 1)
+                          .astype(np.int32))
+    # permutations are permutations
+    for t, a in (("dim", "d_key"), ("item", "i_key"), ("cust", "c_key"), ("nation", "n_key")):
+        k = T.store[(t, a, None)].toArray(np.int32, T.rows[t])
+        assert np.array_equal(np.sort(k), np.arange(T.rows[t], dtype=np.int32)), t
+    # Zipf keys: inverse CDF of 1/rank over 100k values
+    nb = T.rows["build"]
+    bk = T.store[("build", "b_key", None)].toArray(np.int32, nb)
+    ranks = np.arange(1, 100001, dtype=np.float64)
+    cdf = np.cumsum(1.0 / ranks)
+    cdf /= cdf[-1]
+    a, c = datagen.permParams(100000)
+    values = ((a * np.arange(100000, dtype=np.int64) + c) % 100000).astype(np.int32)
+    u = (rnd(seed, cid[("build", "b_key")], np.arange(nb)) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    r = np.minimum(np.searchsorted(cdf, u, side="right"), 99999)
+    assert np.array_equal(bk, values[r])
+    # text rows: 5 vocabulary words, single spaces, space padded to 55 bytes
+    ni = T.rows["item"]
+    offs = T.store[("item", "i_text", "offset")].toArray(np.uint64, ni + 1)
+    chars = T.store[("item", "i_text", "char")].toArray(np.uint8, ni * 55).tobytes()
+    assert np.array_equal(offs, np.arange(ni + 1, dtype=np.uint64) * 55)
+    tc = cid[("item", "i_text")]
+    idx = np.stack([uniform(rnd(seed, tc + j, np.arange(ni)), 0, 91) for j in range(5)], axis=1)
+    for i in range(0, ni, max(1, ni // 500)):
+        words = chars[55 * i:55 * (i + 1)].decode().rstrip(" ").split(" ")
+        assert words == [workloads.VOCAB[k] for k in idx[i]]
+
+
+def test_alone_equals_mixed_and_oracle():
+    print("gpu case: serving alone vs mixed", flush=True)
+    T = tables(0.02)
+    plans = serving.PlanSet()
+    queries = serving.makeMix(40, seed=11, scaleRange=(0.001, 0.02))
+    arena = 4 << 30
+    keep = set(range(len(queries)))
+    recsA, srvA = serving.run(T, plans, queries, 1, arena, keep=keep)
+    recsM, srvM = serving.run(T, plans, queries, 8, arena, keep=keep)
+    overlap = 0
+    for i, q in enumerate(queries):
+        a = serving.resultOf(plans, q.config, recsA[i])
+        m = serving.resultOf(plans, q.config, recsM[i])
+        assert serving.sameResult(q.config, a, m), "query %d %s x%.4f alone != mixed" % (i, q.config, q.scale)
+        want = serving.oracleResult(T, q.config, q.scale)
+        assert serving.matchesOracle(q.config, m, want), "query %d %s x%.4f != oracle" % (i, q.config, q.scale)
+        overlap += sum(1 for j, r in enumerate(recsM) if j != i and r.admit_ns < recsM[i].done_ns
+                       and recsM[i].admit_ns < r.done_ns)
+    assert overlap > 0          # the mixed run did run queries concurrently
+    srvA.destroy()
+    srvM.destroy()
+
+
+def test_admission_respects_arena_and_rejects_oversized():
+    print("gpu case: serving admission", flush=True)
+    T = tables(0.02)
+    plans = serving.PlanSet()
+    queries = [serving.Query("c1", 0.02) for _ in range(12)]
+    need = serving.workspaceBytes(T, plans, "c1", 0.02)
+    r, cols = serving.binding(plans.modules["c1"], T, serving.prefixRows(T, "c1", 0.02))
+    arena = int(need * 2.5)     # room for two at a time
+    recs, srv = serving.run(T, plans, queries, 8, arena)
+    events = sorted([(rc.admit_ns, 1) for rc in recs] + [(rc.done_ns, -1) for rc in recs])
+    live = peak = 0
+    for _, d in events:
+        live += d
+        peak = max(peak, live)
+    assert peak <= 2
+    srv.destroy()
+    small = serving.Server(4, need // 2)
+    qid = small.submit(plans.modules["c1"], r, cols, plans.tunings["c1"])
+    small.start()
+    small.wait(60000)
+    rec = small.record(qid)
+    assert rec.state == 3 and rec.error == -5      # DQC_SERVE_FAILED, DQC_ENOMEM
+    small.destroy()
+
+
+def test_overflow_is_readmitted():
+    print("gpu case: serving retry", flush=True)
+    T = tables(0.02)
+    plans = serving.PlanSet()
+    t = rt.newTuning()
+    C.memmove(C.addressof(t), C.addressof(plans.tunings["c1"]), C.sizeof(t))
+    t.ht_capacity[0] = 16
+    t.ht_capacity[1] = 8
+    plans.tunings["c1"] = t
+    q = serving.Query("c1", 0.02)
+    recs, srv = serving.run(T, plans, [q], 2, 1 << 30, keep={0})
+    assert recs[0].retries > 0
+    got = serving.resultOf(plans, "c1", recs[0])
+    assert serving.matchesOracle("c1", got, serving.oracleResult(T, "c1", 0.02))
+    srv.destroy()
