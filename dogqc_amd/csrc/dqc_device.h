@@ -139,12 +139,26 @@ DQC_DEVICE_INLINE void raise_status(uint32_t* status, uint32_t bit) { atomicOr(s
 // -------------------------------------------------------- keyed hash tables
 // Global tables: tags[cap] (uint64) + structure-of-arrays payload columns.
 
+// Probe sequences are bounded: a key is always within MAX_PROBE slots of its
+// home slot, so a build or group claim that walks farther reports the table
+// as too full (the run is repeated with a larger table), lookups stop there,
+// and an overflowing run never degenerates into a scan of the whole table
+// per row.  Claims also give up early once some row has flagged an
+// overflow in the status word (the run's result is discarded anyway).
+constexpr uint32_t MAX_PROBE = 2048;
+
+DQC_DEVICE_INLINE bool overflow_raised(const uint32_t* status) {
+    return (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ST_HT_OVERFLOW) != 0u;
+}
+
 // Insert a tag; duplicates of an existing tag take the next free slot (the
 // unique-join build of the reference also stores duplicate keys,
 // hashing.cuh:81-100).  Starts at slot s with n slots already tried.
-// Returns the slot, or -1 when the table is full.
-DQC_DEVICE_INLINE int64_t ht_insert_from(uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t s, uint32_t n) {
-    for (; n < cap; ++n) {
+// Returns the slot, or -1 when the table is too full.
+DQC_DEVICE_INLINE int64_t ht_insert_from(uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t s, uint32_t n,
+                                         const uint32_t* status) {
+    for (; n < cap && n < MAX_PROBE; ++n) {
+        if ((n & 31u) == 31u && overflow_raised(status)) return -1;
         uint64_t prev = atomicCAS((unsigned long long*)&tags[s], (unsigned long long)TAG_EMPTY,
                                   (unsigned long long)tag);
         if (prev == TAG_EMPTY) return (int64_t)s;
@@ -153,8 +167,9 @@ DQC_DEVICE_INLINE int64_t ht_insert_from(uint64_t* tags, uint32_t cap, uint64_t 
     return -1;
 }
 
-DQC_DEVICE_INLINE int64_t ht_insert(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
-    return ht_insert_from(tags, cap, tag, slot_of(h, cap), 0u);
+DQC_DEVICE_INLINE int64_t ht_insert(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h,
+                                    const uint32_t* status) {
+    return ht_insert_from(tags, cap, tag, slot_of(h, cap), 0u, status);
 }
 
 // Find the next slot holding `tag`, starting at *cursor (initialise the
@@ -164,7 +179,7 @@ DQC_DEVICE_INLINE uint32_t probe_start(uint64_t h, uint32_t cap) { return slot_o
 
 DQC_DEVICE_INLINE int64_t ht_probe(const uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t& cursor,
                                    uint32_t& visited) {
-    while (visited < cap) {
+    while (visited < cap && visited < MAX_PROBE) {
         uint32_t s = cursor;
         uint64_t t = tags[s];
         cursor = next_slot(s, cap);
@@ -222,8 +237,10 @@ DQC_DEVICE_INLINE int64_t probe_in_window(const probe_win& w, uint32_t s, uint32
 // Group slot in exact mode: the tag is the key, so find-or-claim is one CAS
 // per probed slot with no lock.  Starts at slot s with n slots tried.
 // Returns -1 when the table is full.
-DQC_DEVICE_INLINE int64_t agg_slot_exact_from(uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t s, uint32_t n) {
-    for (; n < cap; ++n) {
+DQC_DEVICE_INLINE int64_t agg_slot_exact_from(uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t s, uint32_t n,
+                                              const uint32_t* status) {
+    for (; n < cap && n < MAX_PROBE; ++n) {
+        if ((n & 31u) == 31u && overflow_raised(status)) return -1;
         uint64_t t = __hip_atomic_load(&tags[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (t == tag) return (int64_t)s;
         if (t == TAG_EMPTY) {
@@ -236,8 +253,9 @@ DQC_DEVICE_INLINE int64_t agg_slot_exact_from(uint64_t* tags, uint32_t cap, uint
     return -1;
 }
 
-DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h) {
-    return agg_slot_exact_from(tags, cap, tag, slot_of(h, cap), 0u);
+DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h,
+                                         const uint32_t* status) {
+    return agg_slot_exact_from(tags, cap, tag, slot_of(h, cap), 0u, status);
 }
 
 // Group slot in hashed mode.  The claiming lane CASes EMPTY->LOCKED, writes
@@ -248,10 +266,11 @@ DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t 
 // lane always observe the publication on their next loop trip.
 template <typename WriteKeys, typename KeysEqual>
 DQC_DEVICE_INLINE int64_t agg_slot_hashed(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h,
-                                          WriteKeys write_keys, KeysEqual keys_equal) {
+                                          const uint32_t* status, WriteKeys write_keys, KeysEqual keys_equal) {
     uint32_t s = slot_of(h, cap);
     uint32_t n = 0;
-    while (n < cap) {
+    while (n < cap && n < MAX_PROBE) {
+        if ((n & 31u) == 31u && overflow_raised(status)) return -1;
         uint64_t t = __hip_atomic_load(&tags[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (t == TAG_EMPTY) {
             uint64_t expected = TAG_EMPTY;
@@ -548,6 +567,42 @@ __global__ void __launch_bounds__(SCAN_BS) dqc_mm_scan_apply(uint32_t* dir, uint
         }
         run += v[i];
     }
+}
+
+// Chunk directory of a push-down join's long-list queue (single
+// workgroup): item i, holding len[i] matches, owns chunks
+// [cstart[i], cstart[i+1]) of `chunk` matches each; cstart[n] and *total are
+// the number of chunks.  n = min(*count, cap) items were queued.
+__global__ void __launch_bounds__(SCAN_BS) dqc_split_scan(const uint32_t* len, const uint64_t* count, uint64_t cap,
+                                                          uint32_t chunk, uint64_t* cstart, uint64_t* total) {
+    __shared__ uint64_t sh[SCAN_BS / WAVE + 1];
+    const uint64_t n = *count < cap ? *count : cap;
+    uint64_t carry = 0;
+    for (uint64_t base = 0; base < n; base += SCAN_BS) {
+        const uint64_t ix = base + threadIdx.x;
+        const uint64_t v = ix < n ? ((uint64_t)len[ix] + chunk - 1u) / chunk : 0u;
+        uint64_t t;
+        const uint64_t ex = block_excl_scan(v, sh, t);
+        if (ix < n) cstart[ix] = carry + ex;
+        carry += t;
+    }
+    if (threadIdx.x == 0) {
+        cstart[n] = carry;
+        *total = carry;
+    }
+}
+
+// Item owning chunk c: the last i < n with cstart[i] <= c.
+DQC_DEVICE_INLINE uint32_t split_item(const uint64_t* cstart, uint64_t n, uint64_t c) {
+    uint64_t lo = 0, hi = n;   // cstart[lo] <= c < cstart[hi]
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (cstart[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return (uint32_t)lo;
 }
 
 }  // namespace dqc

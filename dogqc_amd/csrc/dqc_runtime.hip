@@ -23,9 +23,9 @@
 #include "dogqc_amd.h"
 #include "dqc_runtime_internal.h"
 
-static_assert(sizeof(dqc_plan_tuning) == 1256, "dqc_plan_tuning layout");
-static_assert(sizeof(dqc_plan_status) == 344, "dqc_plan_status layout");
-static_assert(sizeof(dqc_plan_layout) == 1648, "dqc_plan_layout layout");
+static_assert(sizeof(dqc_plan_tuning) == 1640, "dqc_plan_tuning layout");
+static_assert(sizeof(dqc_plan_status) == 600, "dqc_plan_status layout");
+static_assert(sizeof(dqc_plan_layout) == 1904, "dqc_plan_layout layout");
 static_assert(sizeof(dqc_plan_inputs) == 24, "dqc_plan_inputs layout");
 
 using dqcrt::Module;
@@ -72,6 +72,12 @@ bool adjustForRetry(const dqc_plan_layout& L, const dqc_plan_status& st, dqc_pla
     for (int i = 0; i < L.ntemps && i < DQC_MAX_TEMPS; ++i) {
         if (st.temp_count[i] > (uint64_t)L.temp_capacity[i]) {
             tun->temp_capacity[i] = (int64_t)st.temp_count[i];
+            retry = true;
+        }
+    }
+    for (int i = 0; i < L.nhts; ++i) {
+        if (L.split_capacity[i] > 0 && st.split_count[i] > (uint64_t)L.split_capacity[i]) {
+            tun->split_capacity[i] = (int64_t)st.split_count[i];
             retry = true;
         }
     }

@@ -20,6 +20,15 @@ push-down are designed for 64-lane wavefronts:
     that tuple's registers by a lane shuffle.  Every lane does useful work in
     every trip except the last, however skewed the lists (the reference packs
     whole lists into 32 lanes with a leader loop, hashJoins.py:599-681).
+  * long lists across wavefronts: push-down balances the lanes of one
+    wavefront, but a skewed key (Zipf) can still give one wavefront
+    thousands of trips while the rest of the GPU idles.  Probe rows whose
+    list exceeds a threshold are queued (their live registers, offset and
+    length) instead of walked; after the pass a one-workgroup scan cuts the
+    queued lists into chunks of SPLIT_CHUNK matches, and a second launch of
+    the same kernel (PHASE 1) gives every lane one chunk and runs the same
+    push-down walk and operators over them, spreading a hot key over the
+    whole GPU.
 
 Extension over the reference: additional non-equality join conditions are
 evaluated for INNER joins too (the reference evaluates them only for
@@ -36,6 +45,8 @@ from dogqc_amd.hiplang import CType, PACK_BITS
 class EquiJoinTranslator(BinaryTranslator):
 
     usePushDownJoin = False
+    SPLIT_CHUNK = 64                 # matches per queued chunk
+    DEFAULT_SPLIT_THRESHOLD = 1024   # list length above which a probe row is queued
 
     def __init__(self, algExpr, leftChild, rightChild):
         BinaryTranslator.__init__(self, algExpr, leftChild, rightChild)
@@ -161,7 +172,7 @@ class EquiJoinTranslator(BinaryTranslator):
             with pipe.rowLoop():
                 pipe.code.open("if (active[u])")
                 pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : dqc::ht_insert_from(%s, %s, %s, "
-                          "dqc::next_slot(%s, %s), 1u);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
+                          "dqc::next_slot(%s, %s), 1u, &st->flags);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
                 pipe.code.open("if (js >= 0)")
                 for f, att in zip(fields, self.stored):
                     pipe.emit("%s[js] = %s;" % (f, pipe.access(att)))
@@ -401,9 +412,10 @@ class EquiJoinTranslator(BinaryTranslator):
             raise RuntimeError("multimatch probes need a one-row-per-thread pipeline")
         probeSide = self.rightChild.algExpr.outRelation
         probeLive = [att for id, att in a.outRelation.items() if id in probeSide]
+        split = self._splitEnabled(pipe)
         pipe.loopOps += 1
         if EquiJoinTranslator.usePushDownJoin and ctxt.innerLoopCount == 0:
-            self._probePushDown(ctxt, pipe, fdir, fcap, fields, posName, probeLive)
+            self._probePushDown(ctxt, pipe, fdir, fcap, fields, posName, probeLive, split)
         else:
             self._probeLoop(ctxt, pipe, fdir, fcap, fields, posName, probeLive)
 
@@ -438,8 +450,95 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.code.close()
         pipe.emit("active[u] = 0;")
 
-    def _probePushDown(self, ctxt, pipe, fdir, fcap, fields, posName, probeLive):
-        """Push-down parallelism for wave64 (see module docstring)."""
+    def _splitEnabled(self, pipe):
+        """Long-list splitting needs the pipeline's first expansion (no lane
+        reassignment before it), no refill marks, and a pipeline that is not
+        itself launched in two phases (a multimap build as its sink)."""
+        from dogqc_amd.hipTranslator import AggregationTranslator, MaterializeTranslator
+        if pipe.innerDepth or pipe.loopOps or pipe.phased or any(l is None for l in pipe.code.lines):
+            return False
+        t = self
+        while t.parent is not None:
+            p = t.parent
+            if isinstance(p, EquiJoinTranslator) and p.leftChild is t and p.algExpr.multimatch:
+                return False
+            if isinstance(p, (AggregationTranslator, MaterializeTranslator)):
+                break
+            t = p
+        return True
+
+    def _splitQueue(self, ctxt, pipe, moff, mlen, live):
+        """PHASE 0: queue the probe rows whose match list is long; PHASE 1:
+        every lane takes one chunk of a queued list (see module docstring)."""
+        a = self.algExpr
+        J = a.opId
+        prog = ctxt.program
+        idx = self.ht["index"]
+        CH = self.SPLIT_CHUNK
+        est = prog.estimate(self.rightChild.algExpr)
+        prog.splitVars["scap%d" % idx] = (
+            "L.scap[%d] = t->split_capacity[%d] > 0 ? t->split_capacity[%d] : std::max<int64_t>(1024, %s / 256);"
+            % (idx, idx, idx, est))
+        cap = "L.scap[%d]" % idx
+        fcap = pipe.param("scap%d" % J, "uint64_t", cap)
+        fthr = pipe.param("sthr%d" % J, "uint32_t", "t->split_threshold[%d] > 0 ? t->split_threshold[%d] : %d"
+                          % (pipe.pipeId, pipe.pipeId, self.DEFAULT_SPLIT_THRESHOLD))
+        blen = prog.buffer("sq%d_len" % J, CType.UINT, cap)
+        boff = prog.buffer("sq%d_off" % J, CType.UINT, cap)
+        flen = pipe.param("sq%d_len" % J, "uint32_t*", "B_" + blen)
+        foff = pipe.param("sq%d_off" % J, "uint32_t*", "B_" + boff)
+        regs = []
+        for k, (r, ct) in enumerate(live.items()):
+            b = prog.buffer("sq%d_r%d" % (J, k), ct, cap)
+            regs.append((r, pipe.param("sq%d_r%d" % (J, k), "%s*" % ct, "B_" + b)))
+        bcs = prog.buffer("sq%d_cstart" % J, CType.UINT64, cap + " + 1")
+        btot = prog.buffer("sq%d_total" % J, CType.UINT64, "1")
+        fcs = pipe.param("sq%d_cstart" % J, "const uint64_t*", "B_" + bcs)
+        ftot = pipe.param("sq%d_total" % J, "const uint64_t*", "B_" + btot)
+        # PHASE 0: queue long lists (wave-aggregated append), walk the rest
+        pipe.code.open("if (active[u] && %s > %s)" % (mlen, fthr))
+        pipe.emit("const uint64_t qm = dqc::ballot(true);")
+        pipe.emit("const int ql = dqc::first_lane(qm);")
+        pipe.emit("unsigned long long qb = 0;")
+        pipe.emit("if ((int)lane == ql) qb = atomicAdd((unsigned long long*)&st->split_count[%d], "
+                  "(unsigned long long)dqc::popc64(qm));" % idx)
+        pipe.emit("const uint64_t qp = dqc::shfl(qb, ql) + dqc::lanes_below(qm);")
+        pipe.code.open("if (qp < %s)" % fcap)
+        pipe.emit("%s[qp] = (uint32_t)%s;" % (flen, mlen))
+        pipe.emit("%s[qp] = %s;" % (foff, moff))
+        for r, f in regs:
+            pipe.emit("%s[qp] = %s[u];" % (f, r))
+        pipe.code.close()
+        pipe.emit("%s = 0;" % mlen)
+        pipe.code.close()
+        pipe.code.close()   # end of PHASE 0
+        # PHASE 1: the loop index is a chunk of a queued list
+        pipe.code.open("else")
+        pipe.emit("const uint64_t hc = (uint64_t)%s[u];" % pipe.tid)
+        pipe.emit("%s = 0; %s = 0;" % (moff, mlen))
+        pipe.code.open("if (active[u])")
+        pipe.emit("const uint64_t hn = std::min<uint64_t>(st->split_count[%d], %s);" % (idx, fcap))
+        pipe.emit("const uint32_t hi = dqc::split_item(%s, hn, hc);" % fcs)
+        pipe.emit("const uint64_t hk = (hc - %s[hi]) * %du;" % (fcs, CH))
+        for r, f in regs:
+            pipe.emit("%s[u] = %s[hi];" % (r, f))
+        pipe.emit("%s = %s[hi] + (uint32_t)hk;" % (moff, foff))
+        pipe.emit("%s = std::min<uint64_t>(%du, (uint64_t)%s[hi] - hk);" % (mlen, CH, flen))
+        pipe.code.close()
+        pipe.code.close()
+        pipe.phased = True
+        pipe.phaseRows[1] = ("(int64_t)%s * 64" % cap, "*%s" % ftot)
+
+        def scan(prog, idx=idx, blen=blen, bcs=bcs, btot=btot, cap=cap, CH=CH):
+            return ["hipLaunchKernelGGL(dqc::dqc_split_scan, dim3(1), dim3(dqc::SCAN_BS), 0, s, "
+                    "(const uint32_t*)B_%s, (const uint64_t*)&st->split_count[%d], (uint64_t)%s, %du, "
+                    "B_%s, B_%s);" % (blen, idx, cap, CH, bcs, btot)]
+        pipe.phaseBetween = [{"name": "dqc_split_scan", "host": scan}]
+        pipe.annotations.append("SPLIT%d" % J)
+
+    def _probePushDown(self, ctxt, pipe, fdir, fcap, fields, posName, probeLive, split):
+        """Push-down parallelism for wave64 (see module docstring); `split`
+        queues long lists for the second launch."""
         J = self.algExpr.opId
         nw = pipe.bs // 64
         pipe.staticLds.append("__shared__ uint64_t pdincl%d[%d][64];" % (J, nw))
@@ -447,6 +546,10 @@ class EquiJoinTranslator(BinaryTranslator):
         moff = pipe.reg("moff%d" % J, "uint32_t", "0")
         mlen = pipe.reg("mlen%d" % J, "uint64_t", "0")
         pos = posName + "[u]"
+        if split:
+            # everything before this point is the first pass (PHASE 0)
+            pipe.code.lines.insert(0, "    " * (pipe.code.depth - 1) + "if constexpr (PHASE == 0) {")
+            pipe.code.depth += 1
         pipe.emit("%s = 0; %s = 0;" % (moff, mlen))
         pipe.code.open("if (active[u])")
         self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
@@ -459,6 +562,8 @@ class EquiJoinTranslator(BinaryTranslator):
         # tuple moves with it to the lanes that process its matches
         condProbe = list(getattr(self.algExpr, "conditionProbeAttributes", {}).values())
         live = pipe.liveState(probeLive + self.probeKeys + condProbe)
+        if split:
+            self._splitQueue(ctxt, pipe, moff, mlen, live)
         pipe.emit("const uint64_t pincl%d = dqc::wave_incl_scan(%s);" % (J, mlen))
         pipe.emit("const uint64_t ptotal%d = dqc::uniform64(dqc::shfl(pincl%d, 63));" % (J, J))
         pipe.emit("pdincl%d[wave][lane] = pincl%d;" % (J, J))

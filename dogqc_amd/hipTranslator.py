@@ -140,6 +140,8 @@ class Pipeline(object):
         self.attrs = {}
         self.indexRegs = OrderedDict()
         self.phased = False
+        self.phaseBetween = None     # launch steps between PHASE 0 and PHASE 1
+        self.phaseRows = {}          # phase -> (host grid rows, device row count) overrides
         self.dynLdsExpr = None
         self.refillPoints = []
         self.annotations = []
@@ -263,9 +265,13 @@ class Pipeline(object):
         k.emit("(void)st;")
         if self.devRows is not None:
             # rows appended by an earlier kernel of the plan, at most the capacity
-            k.emit("const int64_t nrows = std::min<int64_t>((int64_t)(%s), p.n);" % self.devRows)
+            base = "std::min<int64_t>((int64_t)(%s), p.n)" % self.devRows
         else:
-            k.emit("const int64_t nrows = p.n;")
+            base = "p.n"
+        if 1 in self.phaseRows:
+            k.emit("const int64_t nrows = PHASE == 1 ? (int64_t)(%s) : %s;" % (self.phaseRows[1][1], base))
+        else:
+            k.emit("const int64_t nrows = %s;" % base)
         if U == 1:
             k.emit("constexpr int u = 0;")
         for name, (ct, init, perRow) in self.regs.items():
@@ -311,6 +317,7 @@ class Program(object):
         self.hts = []
         self.estimates = OrderedDict()   # opId -> C++ expression
         self.hostVars = OrderedDict()    # extra host layout statements
+        self.splitVars = OrderedDict()   # long-list queue capacities (after the zero fill of L.scap)
         self.result = None
         self.temps = OrderedDict()       # temp table name -> dict(index, columns, est)
         self.charSource = {}             # attr id -> column slot index of its char heap
@@ -457,6 +464,7 @@ class Program(object):
         h.emit("    int64_t mment[%d];" % max(1, len(self.hts)))
         h.emit("    int64_t rcap;")
         h.emit("    int64_t tcap[%d];" % max(1, len(self.temps)))
+        h.emit("    int64_t scap[%d];" % max(1, len(self.hts)))
         h.emit("    int64_t rows[%d];" % max(1, nt))
         h.emit("    uint32_t ldsw[%d];" % max(1, len(self.pipes)))
         h.emit("    uint32_t ldsr[%d];" % max(1, len(self.pipes)))
@@ -476,6 +484,9 @@ class Program(object):
             h.emit("const int64_t est_%d = std::max<int64_t>(0, %s);" % (opId, expr))
             h.emit("(void)est_%d;" % opId)
         for name, stmt in self.hostVars.items():
+            h.emit(stmt)
+        h.emit("for (int i = 0; i < %d; ++i) L.scap[i] = 0;" % max(1, len(self.hts)))
+        for name, stmt in self.splitVars.items():
             h.emit(stmt)
         h.emit("uint64_t off = align256(sizeof(dqc_plan_status));")
         for i, (name, b) in enumerate(self.buffers.items()):
@@ -515,8 +526,8 @@ class Program(object):
         h.emit("out->ws_bytes = L.bytes;")
         h.emit("out->nsteps = %d;" % len(self.steps))
         h.emit("out->nhts = %d;" % len(self.hts))
-        h.emit("for (int i = 0; i < %d; ++i) { out->ht_capacity[i] = L.htcap[i]; out->mm_entries[i] = L.mment[i]; }"
-               % len(self.hts))
+        h.emit("for (int i = 0; i < %d; ++i) { out->ht_capacity[i] = L.htcap[i]; out->mm_entries[i] = L.mment[i]; "
+               "out->split_capacity[i] = L.scap[i]; }" % len(self.hts))
         h.emit("out->result_capacity = L.rcap;")
         res = self.result["columns"] if self.result else []
         h.emit("out->nresult = %d;" % len(res))
@@ -842,11 +853,13 @@ def closePipeline(ctxt, pipe):
     finishRefills(pipe)
     prog = ctxt.program
     if pipe.phased:
-        ht = pipe.phasedHt
+        between = pipe.phaseBetween if pipe.phaseBetween is not None else mmScanSteps(pipe.phasedHt)
         prog.steps.append({"name": pipe.name + "<0>", "host": lambda p, pipe=pipe: pipelineLaunch(p, pipe, 0)})
-        for s in mmScanSteps(ht):
+        for s in between:
             prog.steps.append(s)
-        prog.steps.append({"name": pipe.name + "<1>", "host": lambda p, pipe=pipe: pipelineLaunch(p, pipe, 1)})
+        rows1 = pipe.phaseRows[1][0] if 1 in pipe.phaseRows else None
+        prog.steps.append({"name": pipe.name + "<1>",
+                           "host": lambda p, pipe=pipe, r=rows1: pipelineLaunch(p, pipe, 1, rowsExpr=r)})
     else:
         prog.steps.append({"name": pipe.name, "host": lambda p, pipe=pipe: pipelineLaunch(p, pipe)})
     ctxt.pipe = None
@@ -1219,7 +1232,7 @@ class AggregationTranslator(UnaryTranslator):
             # pass 2: resolve collisions / claim slots, update the group
             with pipe.rowLoop():
                 pipe.code.open("if (%s)" % gneed)
-                pipe.emit("const int64_t gs = %s == %s ? (int64_t)%s : dqc::agg_slot_exact_from(%s, %s, %s, %s, 0u);"
+                pipe.emit("const int64_t gs = %s == %s ? (int64_t)%s : dqc::agg_slot_exact_from(%s, %s, %s, %s, 0u, &st->flags);"
                           % (gt0, gtag, gs0, ftags, fcap, gtag, gs0))
                 pipe.code.open("if (gs >= 0)")
                 for acc, v in zip(self.accs, vals):
@@ -1246,7 +1259,7 @@ class AggregationTranslator(UnaryTranslator):
                 pipe.emit("const uint64_t gtag = dqc::TAG_TOP | (gh >> 1);")
                 writes = " ".join("%s[s] = %s;" % (f, pipe.access(g)) for f, g in zip(keyFields, groups))
                 eqs = " && ".join(_keyEq(g, "%s[s]" % f, pipe.access(g)) for f, g in zip(keyFields, groups))
-                pipe.emit("const int64_t gs = dqc::agg_slot_hashed(%s, %s, gtag, gh, [&](uint32_t s) { %s }, "
+                pipe.emit("const int64_t gs = dqc::agg_slot_hashed(%s, %s, gtag, gh, &st->flags, [&](uint32_t s) { %s }, "
                           "[&](uint32_t s) { return %s; });" % (ftags, fcap, writes, eqs))
                 pipe.code.open("if (gs >= 0)")
                 for acc, v in zip(self.accs, vals):
@@ -1328,7 +1341,7 @@ class AggregationTranslator(UnaryTranslator):
             ep.emit("%s f%d = lacc%d[s * p.lr];" % (acc["ctype"], acc["i"], acc["i"]))
             ep.emit("for (uint32_t r = 1; r < p.lr; ++r) { const %s v = lacc%d[s * p.lr + r]; %s }"
                     % (acc["ctype"], acc["i"], _combine(acc["kind"], "f%d" % acc["i"], "v")))
-        ep.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, tg, dqc::mix64(tg));" % (ftags, fcap))
+        ep.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, tg, dqc::mix64(tg), &st->flags);" % (ftags, fcap))
         ep.open("if (gs >= 0)")
         for acc in self.accs:
             ep.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], "f%d" % acc["i"]))
@@ -1347,7 +1360,7 @@ class AggregationTranslator(UnaryTranslator):
                 acc["ctype"], acc["i"], acc["i"], _cIdentity(acc["kind"], acc["ctype"])))
             ep.emit("f%d = dqc::%s(f%d);" % (acc["i"], red, acc["i"]))
         ep.open("if (lane == 0)")
-        ep.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, tg, dqc::mix64(tg));" % (ftags, fcap))
+        ep.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, tg, dqc::mix64(tg), &st->flags);" % (ftags, fcap))
         ep.open("if (gs >= 0)")
         for acc in self.accs:
             ep.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], "f%d" % acc["i"]))

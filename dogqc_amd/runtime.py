@@ -41,13 +41,14 @@ class Tuning(C.Structure):
                 ("lds_slots", C.c_int32 * MAX_PIPES), ("lds_replicas", C.c_int32 * MAX_PIPES),
                 ("ht_factor", C.c_float * MAX_HTS), ("ht_capacity", C.c_int64 * MAX_HTS),
                 ("mm_entries", C.c_int64 * MAX_HTS), ("result_capacity", C.c_int64),
-                ("temp_capacity", C.c_int64 * MAX_TEMPS), ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("temp_capacity", C.c_int64 * MAX_TEMPS), ("split_capacity", C.c_int64 * MAX_HTS),
+                ("split_threshold", C.c_int32 * MAX_PIPES), ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class Status(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("ht_overflow_mask", C.c_uint32), ("mm_overflow_mask", C.c_uint32),
                 ("reserved", C.c_uint32), ("result_count", C.c_uint64), ("mm_total", C.c_uint64 * MAX_HTS),
-                ("temp_count", C.c_uint64 * MAX_TEMPS)]
+                ("temp_count", C.c_uint64 * MAX_TEMPS), ("split_count", C.c_uint64 * MAX_HTS)]
 
 
 class Layout(C.Structure):
@@ -56,7 +57,7 @@ class Layout(C.Structure):
                 ("result_capacity", C.c_int64), ("nresult", C.c_int32), ("nprofile", C.c_int32),
                 ("result_offset", C.c_uint64 * MAX_RESULT_COLS), ("profile_offset", C.c_uint64),
                 ("ntemps", C.c_int32), ("ntempcols", C.c_int32), ("temp_capacity", C.c_int64 * MAX_TEMPS),
-                ("temp_offset", C.c_uint64 * MAX_RESULT_COLS)]
+                ("temp_offset", C.c_uint64 * MAX_RESULT_COLS), ("split_capacity", C.c_int64 * MAX_HTS)]
 
 
 class QueryResult(C.Structure):

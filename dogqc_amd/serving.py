@@ -155,9 +155,11 @@ def poissonArrivals(queries, qps, seed=5):
         q.arrival_ns = int(a * 1e9)
 
 
-def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600):
+def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600, log=None):
     """Run queries through a server with `slots` slots; returns (records,
-    server) -- the server stays alive so kept results can be read."""
+    server) -- the server stays alive so kept results can be read.  With
+    `log`, reports how many queries have finished whenever that changes
+    (checked every 5 s)."""
     srv = Server(slots, arena_bytes)
     ids = []
     for i, q in enumerate(queries):
@@ -166,7 +168,20 @@ def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600):
         ids.append(srv.submit(plans.modules[q.config], r, cols, plans.tunings[q.config], q.arrival_ns,
                               keep=i in keep))
     srv.start()
-    srv.wait(int(timeout_s * 1000))
+    deadline = time.perf_counter() + timeout_s
+    while True:
+        rc = rt.lib().dqc_serve_wait(srv.handle, 5000)
+        if rc == 0:
+            break
+        if rc != -1 or time.perf_counter() > deadline:    # -1: still running after 5 s
+            rt.check(rc, "dqc_serve_wait")
+            raise rt.DqcError("serving run did not finish within %d s" % timeout_s)
+        if log:
+            states = [srv.record(i).state for i in ids]
+            running = [i for i, st in enumerate(states) if st == 1]
+            log("%d of %d queries finished (%d slots); running: %s" % (
+                sum(1 for st in states if st >= 2), len(ids), slots,
+                ", ".join("%s x%.2f" % (queries[i].config, queries[i].scale) for i in running[:4])))
     recs = [srv.record(i) for i in ids]
     for i, rec in enumerate(recs):
         if rec.state != 2:
@@ -271,7 +286,8 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     plans = PlanSet()
     arena = int(arena_gb * (1 << 30))
     # warm up every module once (code objects load at their first launch)
-    run(tables, plans, [Query(c, 0.1) for c in CONFIG_PLANS], 1, arena)[1].destroy()
+    run(tables, plans, [Query(c, 0.1) for c in CONFIG_PLANS], 1, arena, log=say)[1].destroy()
+    say("modules loaded and warmed up")
     queries = makeMix(nqueries, seed)
     # the verified sample: the smallest queries of each config, so the CPU
     # oracle over host copies of their prefixes stays within seconds
@@ -284,10 +300,14 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     # capacity: every query arrives at t = 0
     for q in queries:
         q.arrival_ns = 0
-    recsS, srvS = run(tables, plans, queries, 1, arena, keep=set(sample))
+    recsS, srvS = run(tables, plans, queries, 1, arena, keep=set(sample), log=say)
+    say("slowest alone: " + ", ".join("%s x%.2f %.1f ms" % (queries[i].config, queries[i].scale,
+                                                             (recsS[i].done_ns - recsS[i].admit_ns) * 1e-6)
+                                       for i in sorted(range(nqueries), key=lambda i: recsS[i].admit_ns -
+                                                       recsS[i].done_ns)[:6]))
     out["serial"] = latencyStats(recsS)
     say("serial capacity: %.1f q/s" % out["serial"]["qps"])
-    recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample))
+    recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample), log=say)
     out["batched"] = latencyStats(recsB)
     say("batched capacity: %.1f q/s" % out["batched"]["qps"])
     # alone vs mixed, and the CPU oracle, on the sample
@@ -306,10 +326,10 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     # latency at a fixed offered rate
     rate = load * out["serial"]["qps"]
     poissonArrivals(queries, rate, seed + 1)
-    recs, srv = run(tables, plans, queries, 1, arena)
+    recs, srv = run(tables, plans, queries, 1, arena, log=say)
     srv.destroy()
     out["serial_at_rate"] = latencyStats(recs)
-    recs, srv = run(tables, plans, queries, slots, arena)
+    recs, srv = run(tables, plans, queries, slots, arena, log=say)
     srv.destroy()
     out["batched_at_rate"] = latencyStats(recs)
     out["offered_qps"] = rate

@@ -120,6 +120,27 @@ def test_c2_refill_thresholds(thr):
     check("c2_both", workloadAcc("c2", {"build": 5000, "probe": 100000}, seed=34), tuning=t, label="thr=%d" % thr)
 
 
+@pytest.mark.parametrize("thr", [1, 7, 100000])
+def test_c2_split_thresholds(thr):
+    """Long match lists queued for the second launch at any threshold (1:
+    nearly every list, 100000: none) give the same result."""
+    t = rt.newTuning()
+    for i in range(rt.MAX_PIPES):
+        t.split_threshold[i] = thr
+    check("c2_pushdown", workloadAcc("c2", {"build": 20000, "probe": 200000}, seed=35), tuning=t,
+          label="split threshold %d" % thr)
+
+
+def test_c2_split_queue_overflow_retry():
+    t = rt.newTuning()
+    for i in range(rt.MAX_PIPES):
+        t.split_threshold[i] = 2
+    t.split_capacity[0] = 3
+    _, retries = check("c2_pushdown", workloadAcc("c2", {"build": 20000, "probe": 200000}, seed=36), tuning=t,
+                       label="tiny split queue")
+    assert retries > 0
+
+
 def c2ClosedForm(acc):
     """COUNT and SUM(b_pay * p_pay) of c2 without enumerating pairs:
     per key k, count = nb(k) * np(k) and sum = Sb(k) * Sp(k)."""
