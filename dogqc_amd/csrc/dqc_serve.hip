@@ -12,6 +12,10 @@
 //      ones for at most bypass_ms,
 //   3. enqueues an admitted plan, its status readback and an event on the
 //      slot's stream.
+// With size feedback on, a query that needed larger tables teaches the
+// server the sizes per input row for its plan module, and later queries of
+// the module start with tables scaled to their own inputs, so they neither
+// retry nor get admitted with too small a footprint.
 // The client thread only submits requests and reads records.
 #include <hip/hip_runtime.h>
 
@@ -101,12 +105,29 @@ struct Request {
     bool hasWs = false;
     int64_t blockedSince = -1;   // first time it waited for memory at the head of the queue
     bool keep = false;
+    bool fedBack = false;        // sizes from earlier queries applied
 
     void bindInputs() {
         inputs.table_rows = rows.data();
         inputs.cols = cols.data();
     }
 };
+
+// Table sizes per driving input row learned for one plan module.
+struct Feedback {
+    double ht[DQC_MAX_HTS] = {};
+    double mm[DQC_MAX_HTS] = {};
+    double split[DQC_MAX_HTS] = {};
+    double temp[DQC_MAX_TEMPS] = {};
+    double result = 0;
+};
+
+// the largest input table: the size the plan's tables grow with
+int64_t drivingRows(const Request& r) {
+    int64_t m = 1;
+    for (int64_t v : r.rows) m = std::max(m, v);
+    return m;
+}
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -128,6 +149,7 @@ struct Server {
     std::deque<int64_t> pending;          // not yet admitted, in submit order
     int64_t unfinished = 0;
     std::vector<int64_t> toRelease;       // kept workspaces the client released
+    std::map<Module*, Feedback> feedback;
 
     std::thread worker;
     std::atomic<bool> started{false};
@@ -152,9 +174,52 @@ struct Server {
         r.rec.ws = nullptr;
     }
 
+    // Sizes for a first admission from what earlier queries of the module
+    // needed, scaled to this query's inputs; only ever enlarges.
+    void applyFeedback(Request& r) {
+        auto it = feedback.find(r.module);
+        if (it == feedback.end()) return;
+        const Feedback& f = it->second;
+        dqc_plan_layout L;
+        if (r.module->layout(&r.inputs, &r.tuning, &L)) return;
+        const double rows = (double)drivingRows(r);
+        auto grow = [rows](int64_t& field, int64_t current, double ratio, double margin) {
+            if (field > 0 || ratio <= 0) return;
+            const int64_t want = (int64_t)(ratio * rows * margin) + 1;
+            if (want > current) field = want;
+        };
+        for (int i = 0; i < L.nhts && i < DQC_MAX_HTS; ++i) {
+            grow(r.tuning.ht_capacity[i], L.ht_capacity[i], f.ht[i], 1.0);
+            grow(r.tuning.mm_entries[i], L.mm_entries[i], f.mm[i], 1.1);
+            if (L.split_capacity[i] > 0) grow(r.tuning.split_capacity[i], L.split_capacity[i], f.split[i], 1.1);
+        }
+        for (int i = 0; i < L.ntemps && i < DQC_MAX_TEMPS; ++i)
+            grow(r.tuning.temp_capacity[i], L.temp_capacity[i], f.temp[i], 1.1);
+        if (L.nresult > 0) grow(r.tuning.result_capacity, L.result_capacity, f.result, 1.1);
+    }
+
+    // After a query that had to enlarge tables: remember the sizes it ended with.
+    void learn(const Request& r) {
+        Feedback& f = feedback[r.module];
+        const double rows = (double)drivingRows(r);
+        const dqc_plan_tuning& t = r.tuning;
+        for (int i = 0; i < DQC_MAX_HTS; ++i) {
+            if (t.ht_capacity[i] > 0) f.ht[i] = std::max(f.ht[i], t.ht_capacity[i] / rows);
+            if (t.mm_entries[i] > 0) f.mm[i] = std::max(f.mm[i], t.mm_entries[i] / rows);
+            if (t.split_capacity[i] > 0) f.split[i] = std::max(f.split[i], t.split_capacity[i] / rows);
+        }
+        for (int i = 0; i < DQC_MAX_TEMPS; ++i)
+            if (t.temp_capacity[i] > 0) f.temp[i] = std::max(f.temp[i], t.temp_capacity[i] / rows);
+        if (t.result_capacity > 0) f.result = std::max(f.result, t.result_capacity / rows);
+    }
+
     // Admission of one request on slot `si`; false when it does not fit now.
     bool tryAdmit(int64_t id, int si, int64_t t) {
         Request& r = *reqs[id];
+        if (cfg.size_feedback && r.rec.retries == 0 && !r.fedBack) {
+            applyFeedback(r);
+            r.fedBack = true;
+        }
         dqc_plan_layout L;
         int rc = r.module->layout(&r.inputs, &r.tuning, &L);
         if (rc) {
@@ -227,6 +292,7 @@ struct Server {
                 continue;
             }
             if (!r.keep) freeWs(r);
+            if (cfg.size_feedback && r.rec.retries > 0) learn(r);
             finish(r, DQC_SERVE_DONE, 0);
         }
         // 2. admissions in arrival order (stamped after the completions of

@@ -281,11 +281,19 @@ class EquiJoinTranslator(BinaryTranslator):
             out.append(pipe.param("jht%d_p%d" % (J, k), "const %s*" % ct, "B_" + b))
         return out
 
-    def _bindBuild(self, pipe, fields, indexReg):
+    def _bindBuild(self, pipe, fields, indexReg, nullReg=None):
         """Build-side attributes become lazy loads from the stored columns at
-        the row's position ``indexReg[u]``."""
+        the row's position ``indexReg[u]``; with ``nullReg`` (outer join) a
+        null row reads 0 / the empty string instead."""
         for f, att, (b, ct) in zip(fields, self.stored, self.storedBuffers):
-            pipe.setLazy(att, "%s[%s[u]]" % (f, indexReg), [indexReg], ct)
+            load = "%s[%s[u]]" % (f, indexReg)
+            if nullReg is not None:
+                zero = "dqc::str_t{nullptr, nullptr}" if ct == CType.STR_TYPE else "(%s)0" % ct
+                load = "(%s[u] ? %s : %s)" % (nullReg, zero, load)
+                deps = [indexReg, nullReg]
+            else:
+                deps = [indexReg]
+            pipe.setLazy(att, load, deps, ct)
 
     def _matchCond(self, ctxt, pipe, fields, indexReg):
         """C expression: stored keys at indexReg[u] equal the probe keys and
@@ -406,22 +414,27 @@ class EquiJoinTranslator(BinaryTranslator):
                 pipe.code.close()
             self.parent.consume(ctxt)
             return
-        if a.joinType == Join.OUTER:
-            raise NotImplementedError("outer joins are not supported by the HIP backend yet")
         if pipe.U != 1:
             raise RuntimeError("multimatch probes need a one-row-per-thread pipeline")
         probeSide = self.rightChild.algExpr.outRelation
         probeLive = [att for id, att in a.outRelation.items() if id in probeSide]
         split = self._splitEnabled(pipe)
         pipe.loopOps += 1
-        if EquiJoinTranslator.usePushDownJoin and ctxt.innerLoopCount == 0:
+        if a.joinType == Join.OUTER:
+            # the probe row's own lane tracks whether it found a match, so
+            # outer joins walk lists per lane
+            self._probeLoop(ctxt, pipe, fdir, fcap, fields, posName, probeLive, outer=True)
+        elif EquiJoinTranslator.usePushDownJoin and ctxt.innerLoopCount == 0:
             self._probePushDown(ctxt, pipe, fdir, fcap, fields, posName, probeLive, split)
         else:
             self._probeLoop(ctxt, pipe, fdir, fcap, fields, posName, probeLive)
 
-    def _probeLoop(self, ctxt, pipe, fdir, fcap, fields, posName, probeLive):
+    def _probeLoop(self, ctxt, pipe, fdir, fcap, fields, posName, probeLive, outer=False):
         """Each lane walks its own match list; the wavefront loops while any
-        lane has matches left (hashJoins.py:465-534)."""
+        lane has matches left (hashJoins.py:465-534).  An outer join gives a
+        probe row without a match one more trip as a null row: its build
+        attributes read 0 / empty and carry a null flag that aggregations and
+        materialisation honour (hashJoins.py:473-530; cudaTranslator.py:475)."""
         J = self.algExpr.opId
         moff = pipe.reg("moff%d" % J, "uint32_t", "0")
         mend = pipe.reg("mend%d" % J, "uint32_t", "0")
@@ -435,11 +448,27 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.code.close()
         pipe.need(probeLive)
         pipe.emit("int mact%d = active[u] && %s < %s;" % (J, moff, mend))
-        pipe.code.open("while (dqc::ballot(mact%d))" % J)
-        pipe.emit("%s = %s;" % (pos, moff))
-        self._bindBuild(pipe, fields, posName)
+        nullReg = None
+        if outer:
+            nullReg = "onl%d" % J
+            pipe.reg(nullReg, "int", "0")
+            pipe.indexRegs[nullReg] = "int"
+            opend = "opend%d" % J
+            pipe.emit("int %s = active[u];   // probe row still owes its null row" % opend)
+            pipe.code.open("while (dqc::ballot(mact%d || %s))" % (J, opend))
+            pipe.emit("%s[u] = !mact%d && %s;" % (nullReg, J, opend))
+            pipe.emit("%s = mact%d ? %s : 0u;" % (pos, J, moff))
+            for att in self.stored:
+                pipe.nullFlags[att.id] = nullReg + "[u]"
+        else:
+            pipe.code.open("while (dqc::ballot(mact%d))" % J)
+            pipe.emit("%s = %s;" % (pos, moff))
+        self._bindBuild(pipe, fields, posName, nullReg)
         pipe.emit("active[u] = mact%d;" % J)
         pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, posName))
+        if outer:
+            pipe.emit("if (active[u]) %s = 0;" % opend)
+            pipe.emit("if (%s[u]) { active[u] = 1; %s = 0; }" % (nullReg, opend))
         ctxt.innerLoopCount += 1
         pipe.innerDepth += 1
         self.parent.consume(ctxt)

@@ -147,6 +147,7 @@ class Pipeline(object):
         self.annotations = []
         self.innerDepth = 0
         self.probedJoins = []
+        self.nullFlags = {}         # attribute id -> null flag (outer-join build attributes)
         self.loopOps = 0            # operators so far that reassign tuples to lanes
         self.param("n", "int64_t", source["rows"])
         self.devRows = source.get("devRows")   # device-side row count (temp-table scans)
@@ -241,6 +242,9 @@ class Pipeline(object):
             else:
                 for d in st.deps:
                     regs[d] = self.indexRegs.get(d) or self.regs[d][0]
+            if a.id in self.nullFlags:
+                r = self.nullFlags[a.id][:-3]     # the flag register, without [u]
+                regs[r] = self.regs[r][0]
         return regs
 
     # ---- rendering
@@ -1002,10 +1006,15 @@ class MaterializeTranslator(UnaryTranslator):
             prog.buffer(bname, ct, "L.rcap" if isResult else "L.tcap[%d]" % ti)
             f = pipe.param("%s%d" % (fprefix, i), "%s*" % ct, "B_%s" % bname)
             val = pipe.access(att)
+            null = pipe.nullFlags.get(att.id)
             if ct == CType.STR_OFFS:
                 ci = prog.charSource[att.id]
                 fc = pipe.param("c%d" % ci, "const char*", "C_%d" % ci)
+                if null:    # a null string is the empty string
+                    val = "(%s ? dqc::str_t{%s, %s} : %s)" % (null, fc, fc, val)
                 val = "dqc::to_string_offset(%s, %s)" % (fc, val)
+            elif null and ct != CType.STR_TYPE:
+                val = "(%s ? (%s)0 : %s)" % (null, ct, val)
             cols.append({"name": att.name, "type": att.dataType.name, "ctype": ct, "buffer": bname,
                          "char_col": prog.charSource.get(att.id), "field": f, "val": val})
         public = [{k: v for k, v in c.items() if k not in ("field", "val")} for c in cols]
@@ -1083,8 +1092,13 @@ class AggregationTranslator(UnaryTranslator):
         self.consumeHashTable(ctxt)
 
     def _accumulators(self, ctxt):
+        """Distinct accumulators.  An input that can be null (the build side
+        of an outer join) guards its accumulator: COUNT(attr), SUM, MIN and
+        MAX skip null rows (cudaTranslator.py:475-476 does so for COUNT);
+        AVG divides by COUNT(*) as in the reference."""
         a = self.algExpr
         dec = ctxt.decimalRepr
+        nulls = ctxt.pipe.nullFlags if ctxt.pipe is not None else {}
         self.accs = []
         self.aggAcc = OrderedDict()
         index = {}
@@ -1094,13 +1108,17 @@ class AggregationTranslator(UnaryTranslator):
             kind = {Reduction.SUM: "sum", Reduction.AVG: "sum", Reduction.COUNT: "count",
                     Reduction.MIN: "min", Reduction.MAX: "max"}[red]
             inAttr = a.aggregateInAttributes[inId] if inId is not None else None
-            if kind == "count":
+            null = nulls.get(inAttr.id) if inAttr is not None else None
+            if kind == "count" and null is None:
                 inAttr = None
             key = (kind, inAttr.id if inAttr is not None else None, ct)
             if key not in index:
                 index[key] = len(self.accs)
-                self.accs.append({"kind": kind, "in": inAttr, "ctype": ct, "i": len(self.accs)})
+                self.accs.append({"kind": kind, "in": inAttr, "ctype": ct, "i": len(self.accs), "null": null})
             self.aggAcc[id] = index[key]
+        if a.avgAggregates and not any(c["kind"] == "count" and c["in"] is None for c in self.accs):
+            # AVG's divisor: COUNT(*) even when the plan only counts a nullable attribute
+            self.accs.append({"kind": "count", "in": None, "ctype": CType.INT, "i": len(self.accs), "null": None})
 
     def _fusable(self, pipe, groups):
         """A group-by whose keys are exactly the keys of a unique join probed
@@ -1127,7 +1145,7 @@ class AggregationTranslator(UnaryTranslator):
         prog = ctxt.program
         self._accumulators(ctxt)
         pipe.emit("// aggregation (opId %d)" % a.opId)
-        ins = [acc["in"] for acc in self.accs if acc["in"] is not None]
+        ins = [acc["in"] for acc in self.accs if acc["in"] is not None and acc["kind"] != "count"]
         groups = list(a.groupAttributes.values())
         pipe.need(groups + ins)
         est = prog.estimate(a)
@@ -1159,7 +1177,7 @@ class AggregationTranslator(UnaryTranslator):
             with pipe.rowLoop():
                 pipe.code.open("if (active[u])")
                 for acc, v in zip(self.accs, vals):
-                    pipe.emit(_combine(acc["kind"], acc["reg"], v))
+                    pipe.emit(_guarded(acc, _combine(acc["kind"], acc["reg"], v)))
                 pipe.code.close()
             # end of kernel: wave reduction + one atomic per wavefront
             for acc in self.accs:
@@ -1217,7 +1235,7 @@ class AggregationTranslator(UnaryTranslator):
                 pipe.code.open("if (ls >= 0)")
                 pipe.emit("const uint32_t li = (uint32_t)ls * p.lr + (lane & (p.lr - 1u));")
                 for acc, v in zip(self.accs, vals):
-                    pipe.emit(_ldsOp(acc["kind"], "&lacc%d[li]" % acc["i"], v))
+                    pipe.emit(_guarded(acc, _ldsOp(acc["kind"], "&lacc%d[li]" % acc["i"], v)))
                 pipe.code.close()
                 pipe.code.open("else")
                 pipe.emit("%s = 1;" % gneed)
@@ -1236,7 +1254,7 @@ class AggregationTranslator(UnaryTranslator):
                           % (gt0, gtag, gs0, ftags, fcap, gtag, gs0))
                 pipe.code.open("if (gs >= 0)")
                 for acc, v in zip(self.accs, vals):
-                    pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
+                    pipe.emit(_guarded(acc, "%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v)))
                 if self.exact64:
                     pipe.emit("if (%s == 0) %s[%s] = 1ull;   // marks the extra slot used" % (gtag, ftags, fcap))
                 pipe.code.close()
@@ -1263,7 +1281,7 @@ class AggregationTranslator(UnaryTranslator):
                           "[&](uint32_t s) { return %s; });" % (ftags, fcap, writes, eqs))
                 pipe.code.open("if (gs >= 0)")
                 for acc, v in zip(self.accs, vals):
-                    pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
+                    pipe.emit(_guarded(acc, "%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v)))
                 pipe.code.close()
                 pipe.emit(overflow)
                 pipe.code.close()
@@ -1287,7 +1305,7 @@ class AggregationTranslator(UnaryTranslator):
             pipe.code.open("if (active[u])")
             pipe.emit("const int64_t gs = %s[u];" % pj["slot"])
             for acc, v in zip(self.accs, vals):
-                pipe.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v))
+                pipe.emit(_guarded(acc, "%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v)))
             pipe.emit("%s[gs] = 1;" % fhit)
             pipe.code.close()
 
@@ -1430,7 +1448,7 @@ class AggregationTranslator(UnaryTranslator):
                     pipe.setLazy(g, "%s[%s]" % (f, tid), [pipe.tid], ct)
         countAcc = None
         for acc in self.accs:
-            if acc["kind"] == "count":
+            if acc["kind"] == "count" and acc["in"] is None:
                 countAcc = acc
         for id, accIdx in self.aggAcc.items():
             attr = a.aggregateAttributes[id]
@@ -1442,6 +1460,13 @@ class AggregationTranslator(UnaryTranslator):
             pipe.setLazy(attr, loader, [pipe.tid], ct)
         self.parent.consume(ctxt)
         closePipeline(ctxt, pipe)
+
+
+def _guarded(acc, stmt):
+    """An accumulator update, skipped for rows whose input is null."""
+    if acc.get("null"):
+        return "if (!(%s)) { %s }" % (acc["null"], stmt)
+    return stmt
 
 
 def _combine(kind, reg, v):

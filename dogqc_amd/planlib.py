@@ -172,6 +172,24 @@ def _featurePlan(name, alg):
         p = alg.aggregation(["t_bq"], [(Reduction.SUM, "t_f", "sf"), (Reduction.COUNT, "", "c")],
                             alg.map("t_bq", scal.DivExpr(_A("t_big"), _C("1073741824", Type.INT64)), alg.scan("t")))
         return p, {3: {"numgroups": 2100}}
+    if name == "f_outer":
+        # TPC-H Q13 shape (query/plans/tpch13.py): probe rows without a
+        # match count 0 through COUNT(build attribute)
+        p = alg.aggregation(["t_grp"], [(Reduction.COUNT, "u_id", "c"), (Reduction.SUM, "u_w", "sw"),
+                                        (Reduction.MAX, "u_w", "mw"), (Reduction.COUNT, "", "n")],
+                            alg.outerjoin([("u_grp", "t_grp")], None,
+                                          alg.selection(scal.LargerExpr(_A("u_w"), _C("2.5", Type.DOUBLE)),
+                                                        alg.scan("u")),
+                                          alg.scan("t")))
+        return p, {5: {"numgroups": 128}}
+    if name == "f_outer_project":
+        # outer join with an extra condition, null build attributes materialised as 0 / ''
+        p = alg.projection(["t_id", "u_id", "u_w", "u_name"],
+                           alg.outerjoin([("u_grp", "t_grp")], scal.LargerExpr(_A("u_w"), _C("4.0", Type.DOUBLE)),
+                                         alg.scan("u"),
+                                         alg.selection(scal.SmallerExpr(_A("t_id"), _C("400", Type.INT)),
+                                                       alg.scan("t"))))
+        return p, {}
     if name == "f_temp":
         # TPC-H Q15 shape (query/plans/tpch15.py): a temp table of per-group
         # sums, the maximum over one scan of it joined back to a second scan,
@@ -199,7 +217,7 @@ def _featurePlan(name, alg):
 
 FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
                  "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
-                 "f_groupby_int64"]
+                 "f_groupby_int64", "f_outer", "f_outer_project"]
 
 # name -> (workload or "feature", options)
 PLANS = OrderedDict()

@@ -67,7 +67,7 @@ class QueryResult(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_retries", C.c_int32), ("arena_bytes", C.c_uint64),
-                ("bypass_ms", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("bypass_ms", C.c_int32), ("size_feedback", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class ServeRequest(C.Structure):

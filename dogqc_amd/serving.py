@@ -32,9 +32,10 @@ SCALE_RANGE = (0.1, 10.0)
 class Server(object):
     """ctypes wrapper of one dqc_serve server."""
 
-    def __init__(self, slots, arena_bytes, max_retries=8, bypass_ms=50):
+    def __init__(self, slots, arena_bytes, max_retries=8, bypass_ms=50, size_feedback=True):
         cfg = rt.ServeConfig()
         cfg.slots, cfg.max_retries, cfg.arena_bytes, cfg.bypass_ms = slots, max_retries, int(arena_bytes), bypass_ms
+        cfg.size_feedback = 1 if size_feedback else 0
         self.handle = C.c_void_p()
         rt.check(rt.lib().dqc_serve_create(C.byref(cfg), C.byref(self.handle)), "dqc_serve_create")
 

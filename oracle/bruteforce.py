@@ -83,12 +83,20 @@ class BruteForce(object):
                         if node.conditions is not None and not self.value(node.conditions, pair):
                             continue
                         matched = True
-                        if node.joinType == Join.INNER:
+                        if node.joinType in (Join.INNER, Join.OUTER):
                             out.append(pair)
                 if node.joinType == Join.SEMI and matched:
                     out.append(p)
                 if node.joinType == Join.ANTI and not matched:
                     out.append(p)
+                if node.joinType == Join.OUTER and not matched:
+                    # build attributes null: value 0 / empty string plus a null marker
+                    row = dict(p)
+                    for id, att in node.leftChild.outRelation.items():
+                        ct = regType(att.dataType, self.decimal)
+                        row[id] = b"" if ct == "str" else np.zeros(1, dtype=ct)[0]
+                        row[("isnull", id)] = True
+                    out.append(row)
             return out
         if isinstance(node, ra.Aggregation):
             rows = self.rows(node.child)
@@ -103,15 +111,16 @@ class BruteForce(object):
                 o = dict(zip(gids, key))
                 for id, (inId, red) in node.aggregateTuples.items():
                     ct = regType(node.aggregateAttributes[id].dataType, self.decimal)
-                    vals = [m[inId] for m in members] if inId is not None else []
+                    # null values (outer join) are skipped
+                    vals = [m[inId] for m in members if not m.get(("isnull", inId))] if inId is not None else []
                     if red == Reduction.COUNT:
-                        o[id] = np.int32(len(members))
+                        o[id] = np.int32(len(members) if inId is None else len(vals))
                     elif red in (Reduction.SUM, Reduction.AVG):
                         if np.dtype(ct).kind == "f":
                             s = sum((Fraction(float(v)) for v in vals), Fraction(0))
                             if red == Reduction.AVG:
-                                # C: the rounded sum divided by the count
-                                o[id] = float(s) / float(len(vals)) if vals else float("nan")
+                                # C: the rounded sum divided by the group's row count
+                                o[id] = float(s) / float(len(members)) if members else float("nan")
                             else:
                                 o[id] = float(s)
                         else:

@@ -13,7 +13,10 @@ It restates the reference's operator semantics operator-at-a-time:
     codegen.py:37-45 (DATE unsigned, FLOAT/DOUBLE -> decimal type);
   * equi-joins (hashJoins.py): INNER yields every matching (build, probe)
     pair, SEMI/ANTI keep probe rows with/without a match (extra join
-    conditions evaluated per pair), with sort-based matching instead of hash
+    conditions evaluated per pair), OUTER adds every probe row without a
+    match once with null build attributes (hashJoins.py:473-530; a null
+    value reads as 0 / empty string, COUNT(attr), SUM, MIN and MAX skip it,
+    cudaTranslator.py:475-476), with sort-based matching instead of hash
     tables;
   * aggregation (cudaTranslator.py:425-525): groups by the key tuple; SUM
     and AVG of doubles use math.fsum (exactly rounded, so the oracle's own
@@ -351,6 +354,25 @@ class Evaluator(object):
             hit[pi] = True
             keep = hit if node.joinType == Join.SEMI else ~hit
             return {k: v[keep] for k, v in R.items()}, int(keep.sum())
+        if node.joinType == Join.OUTER:
+            # probe rows without a match appear once, build attributes null
+            hit = np.zeros(nr, dtype=bool)
+            hit[pi] = True
+            miss = np.nonzero(~hit)[0]
+            out = {}
+            for k, v in pairs.items():
+                if k in R:
+                    out[k] = np.concatenate([v, R[k][miss]])
+                elif isinstance(k, tuple):
+                    out[k] = np.concatenate([v, np.ones(len(miss), dtype=bool)])
+                else:
+                    out[k] = np.concatenate([v, np.zeros(len(miss), dtype=v.dtype) if v.dtype != object
+                                             else np.array([b""] * len(miss), dtype=object)])
+            for id in node.leftChild.outRelation:
+                if id in out and ("isnull", id) not in out:
+                    out[("isnull", id)] = np.concatenate([np.zeros(len(pi), dtype=bool),
+                                                          np.ones(len(miss), dtype=bool)])
+            return out, len(pi) + len(miss)
         raise NotImplementedError(str(node.joinType))
 
     def _Aggregation(self, node):
@@ -374,21 +396,31 @@ class Evaluator(object):
         for id, (inId, red) in node.aggregateTuples.items():
             attr = node.aggregateAttributes[id]
             ct = self.ctype(attr)
+            null = rel.get(("isnull", inId)) if inId is not None else None
+            null = null[order] if null is not None else None
             if red == Reduction.COUNT:
-                out[id] = counts.astype(np.int32)
+                if null is None:
+                    out[id] = counts.astype(np.int32)
+                else:   # COUNT(attr) skips null values
+                    out[id] = np.array([int((~null[bounds[gi]:bounds[gi + 1]]).sum()) for gi in range(ng)],
+                                       dtype=np.int32)
                 continue
             vals = rel[inId][order]
             inCt = self.ctype(node.aggregateInAttributes[inId])
             res = np.empty(ng, dtype=ct)
             for gi in range(ng):
                 seg = vals[bounds[gi]:bounds[gi + 1]]
-                res[gi] = self._reduce(red, seg, inCt, ct)
+                n = len(seg)
+                if null is not None:    # SUM/MIN/MAX/AVG skip null values (AVG still divides by COUNT(*))
+                    seg = seg[~null[bounds[gi]:bounds[gi + 1]]]
+                res[gi] = self._reduce(red, seg, inCt, ct, n)
             out[id] = res
         return out, ng
 
     @staticmethod
-    def _reduce(red, seg, inCt, ct):
+    def _reduce(red, seg, inCt, ct, rows=None):
         kind = np.dtype(ct).kind
+        rows = len(seg) if rows is None else rows
         if red in (Reduction.SUM, Reduction.AVG):
             if kind == "f":
                 if np.dtype(inCt).kind in "iu":
@@ -396,7 +428,7 @@ class Evaluator(object):
                 else:
                     s = math.fsum(seg.astype(np.float64))
                 if red == Reduction.AVG:
-                    return s / float(len(seg)) if len(seg) else float("nan")
+                    return s / float(rows) if rows else float("nan")
                 return s
             # integer sum with two's-complement wrap-around
             s = int(np.sum(seg.astype(np.int64).astype(object))) if len(seg) else 0
