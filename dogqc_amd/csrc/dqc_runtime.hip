@@ -56,7 +56,7 @@ bool adjustForRetry(const dqc_plan_layout& L, const dqc_plan_status& st, dqc_pla
     bool retry = false;
     if (st.flags & DQC_ST_HT_OVERFLOW) {
         for (int i = 0; i < L.nhts; ++i)
-            if (st.ht_overflow_mask & (1u << i)) tun->ht_capacity[i] = std::max<int64_t>(64, L.ht_capacity[i] * 2);
+            if (st.ht_overflow_mask & (1u << i)) tun->ht_capacity[i] = std::max<int64_t>(64, L.ht_capacity[i] * 4);
         retry = true;
     }
     if (st.flags & DQC_ST_MM_OVERFLOW) {

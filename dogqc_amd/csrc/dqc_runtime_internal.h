@@ -20,8 +20,9 @@ struct Module {
 };
 
 // After a run: enlarge in `tun` every table the status reports too small
-// (hash tables double; multimaps, temp tables, long-list queues and the
-// result get their exact counts).  Returns true when the run has to be repeated.
+// (hash tables grow fourfold, since the number of keys that did not fit is
+// unknown; multimaps, temp tables, long-list queues and the result get
+// their exact counts).  Returns true when the run has to be repeated.
 bool adjustForRetry(const dqc_plan_layout& L, const dqc_plan_status& st, dqc_plan_tuning* tun);
 
 }  // namespace dqcrt

@@ -105,7 +105,7 @@ struct Request {
     bool hasWs = false;
     int64_t blockedSince = -1;   // first time it waited for memory at the head of the queue
     bool keep = false;
-    bool fedBack = false;        // sizes from earlier queries applied
+    bool heavy = false;          // counted against max_heavy while running
 
     void bindInputs() {
         inputs.table_rows = rows.data();
@@ -146,7 +146,7 @@ struct Server {
     std::mutex mu;
     std::condition_variable cv;
     std::vector<std::unique_ptr<Request>> reqs;
-    std::deque<int64_t> pending;          // not yet admitted, in submit order
+    std::deque<int64_t> pending;          // not yet admitted: arrived ones first, then by arrival time
     int64_t unfinished = 0;
     std::vector<int64_t> toRelease;       // kept workspaces the client released
     std::map<Module*, Feedback> feedback;
@@ -174,8 +174,8 @@ struct Server {
         r.rec.ws = nullptr;
     }
 
-    // Sizes for a first admission from what earlier queries of the module
-    // needed, scaled to this query's inputs; only ever enlarges.
+    // Sizes from what earlier queries of the module needed, scaled to this
+    // query's inputs; only ever enlarges.
     void applyFeedback(Request& r) {
         auto it = feedback.find(r.module);
         if (it == feedback.end()) return;
@@ -184,9 +184,9 @@ struct Server {
         if (r.module->layout(&r.inputs, &r.tuning, &L)) return;
         const double rows = (double)drivingRows(r);
         auto grow = [rows](int64_t& field, int64_t current, double ratio, double margin) {
-            if (field > 0 || ratio <= 0) return;
+            if (ratio <= 0) return;
             const int64_t want = (int64_t)(ratio * rows * margin) + 1;
-            if (want > current) field = want;
+            if (want > std::max<int64_t>(field, current)) field = want;
         };
         for (int i = 0; i < L.nhts && i < DQC_MAX_HTS; ++i) {
             grow(r.tuning.ht_capacity[i], L.ht_capacity[i], f.ht[i], 1.0);
@@ -198,7 +198,7 @@ struct Server {
         if (L.nresult > 0) grow(r.tuning.result_capacity, L.result_capacity, f.result, 1.1);
     }
 
-    // After a query that had to enlarge tables: remember the sizes it ended with.
+    // After a run that had to enlarge tables: remember the enlarged sizes.
     void learn(const Request& r) {
         Feedback& f = feedback[r.module];
         const double rows = (double)drivingRows(r);
@@ -213,28 +213,49 @@ struct Server {
         if (t.result_capacity > 0) f.result = std::max(f.result, t.result_capacity / rows);
     }
 
-    // Admission of one request on slot `si`; false when it does not fit now.
-    bool tryAdmit(int64_t id, int si, int64_t t) {
+    int runningHeavy() const {
+        int n = 0;
+        for (const Slot& sl : slots)
+            if (sl.req >= 0 && reqs[sl.req]->heavy) ++n;
+        return n;
+    }
+
+    int freeSlots() const {
+        int n = 0;
+        for (const Slot& sl : slots)
+            if (sl.req < 0) ++n;
+        return n;
+    }
+
+    enum Admit { ADMITTED, HELD_HEAVY, HELD_LIGHT, NO_MEMORY };
+
+    // Admission of one request on slot `si`.  HELD_HEAVY: the heavy limit is
+    // reached; HELD_LIGHT: a light query leaves the last free slot to a
+    // waiting heavy one (`heavyWaiting`); NO_MEMORY: the arena is too full.
+    // Failing queries count as admitted (they are finished).
+    Admit tryAdmit(int64_t id, int si, int64_t t, bool heavyWaiting) {
         Request& r = *reqs[id];
-        if (cfg.size_feedback && r.rec.retries == 0 && !r.fedBack) {
-            applyFeedback(r);
-            r.fedBack = true;
-        }
+        // every (re)admission: what any query of the module needed so far
+        if (cfg.size_feedback) applyFeedback(r);
         dqc_plan_layout L;
         int rc = r.module->layout(&r.inputs, &r.tuning, &L);
         if (rc) {
             fail(rc, "plan layout failed (%d)", rc);
             finish(r, DQC_SERVE_FAILED, rc);
-            return true;
+            return ADMITTED;
         }
         if (L.ws_bytes > cfg.arena_bytes) {
             finish(r, DQC_SERVE_FAILED, DQC_ENOMEM);
-            return true;
+            return ADMITTED;
         }
+        const bool heavy = cfg.heavy_bytes > 0 && cfg.max_heavy > 0 && L.ws_bytes >= cfg.heavy_bytes;
+        if (heavy && runningHeavy() >= cfg.max_heavy) return HELD_HEAVY;
+        if (!heavy && heavyWaiting && freeSlots() <= 1) return HELD_LIGHT;
         uint64_t off = 0;
-        if (!arena->alloc(L.ws_bytes, &off)) return false;
+        if (!arena->alloc(L.ws_bytes, &off)) return NO_MEMORY;
         r.arenaOff = off;
         r.hasWs = true;
+        r.heavy = heavy;
         r.rec.ws = (char*)arenaBase + off;
         r.rec.ws_bytes = L.ws_bytes;
         r.rec.layout = L;
@@ -252,10 +273,10 @@ struct Server {
             (void)hipStreamSynchronize(s.stream);
             freeWs(r);
             finish(r, DQC_SERVE_FAILED, rc != DQC_OK ? rc : DQC_EHIP);
-            return true;
+            return ADMITTED;
         }
         s.req = id;
-        return true;
+        return ADMITTED;
     }
 
     // One scheduler turn; returns whether anything happened.
@@ -287,7 +308,10 @@ struct Server {
                 } else {
                     ++r.rec.retries;
                     r.rec.state = DQC_SERVE_PENDING;
-                    enqueuePending(r.id);
+                    // the enlarged sizes already tell later queries of the
+                    // module what they need; the query itself runs again next
+                    if (cfg.size_feedback) learn(r);
+                    pending.push_front(r.id);
                 }
                 continue;
             }
@@ -298,16 +322,24 @@ struct Server {
         // 2. admissions in arrival order (stamped after the completions of
         // this turn, whose slots and memory they may take over)
         const int64_t ta = now();
+        bool heavyWaiting = false;
         for (auto it = pending.begin(); it != pending.end();) {
             int si = freeSlot();
             if (si < 0) break;
             Request& r = *reqs[*it];
             if (r.rec.arrival_ns > ta) break;   // pending is kept in arrival order
-            if (tryAdmit(*it, si, ta)) {
+            const Admit a = tryAdmit(*it, si, ta, heavyWaiting);
+            if (a == ADMITTED) {
                 progress = true;
                 it = pending.erase(it);
                 continue;
             }
+            if (a == HELD_HEAVY) {   // later light queries may still run now
+                heavyWaiting = true;
+                ++it;
+                continue;
+            }
+            if (a == HELD_LIGHT) break;
             // does not fit: wait, or let later queries pass for a while
             if (r.blockedSince < 0) r.blockedSince = ta;
             if (cfg.bypass_ms <= 0 || ta - r.blockedSince > (int64_t)cfg.bypass_ms * 1000000) break;

@@ -67,7 +67,8 @@ class QueryResult(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_retries", C.c_int32), ("arena_bytes", C.c_uint64),
-                ("bypass_ms", C.c_int32), ("size_feedback", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("bypass_ms", C.c_int32), ("size_feedback", C.c_int32), ("max_heavy", C.c_int32),
+                ("heavy_bytes", C.c_uint64), ("reserved", C.c_int32 * 2)]
 
 
 class ServeRequest(C.Structure):

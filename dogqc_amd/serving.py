@@ -16,6 +16,7 @@ a fixed rate (p50/p99 of arrival-to-completion), and checks a sample of the
 results against the CPU oracle and between the alone and mixed runs.
 """
 import ctypes as C
+import json
 import math
 import statistics
 import time
@@ -32,10 +33,12 @@ SCALE_RANGE = (0.1, 10.0)
 class Server(object):
     """ctypes wrapper of one dqc_serve server."""
 
-    def __init__(self, slots, arena_bytes, max_retries=8, bypass_ms=50, size_feedback=True):
+    def __init__(self, slots, arena_bytes, max_retries=8, bypass_ms=50, size_feedback=True, max_heavy=0,
+                 heavy_bytes=0):
         cfg = rt.ServeConfig()
         cfg.slots, cfg.max_retries, cfg.arena_bytes, cfg.bypass_ms = slots, max_retries, int(arena_bytes), bypass_ms
         cfg.size_feedback = 1 if size_feedback else 0
+        cfg.max_heavy, cfg.heavy_bytes = int(max_heavy), int(heavy_bytes)
         self.handle = C.c_void_p()
         rt.check(rt.lib().dqc_serve_create(C.byref(cfg), C.byref(self.handle)), "dqc_serve_create")
 
@@ -156,12 +159,12 @@ def poissonArrivals(queries, qps, seed=5):
         q.arrival_ns = int(a * 1e9)
 
 
-def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600, log=None):
+def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600, log=None, max_heavy=0,
+        heavy_bytes=0):
     """Run queries through a server with `slots` slots; returns (records,
     server) -- the server stays alive so kept results can be read.  With
-    `log`, reports how many queries have finished whenever that changes
-    (checked every 5 s)."""
-    srv = Server(slots, arena_bytes)
+    `log`, reports every 5 s how many queries have finished."""
+    srv = Server(slots, arena_bytes, max_heavy=max_heavy, heavy_bytes=heavy_bytes)
     ids = []
     for i, q in enumerate(queries):
         rows = prefixRows(tables, q.config, q.scale)
@@ -189,6 +192,21 @@ def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600, log=
             raise rt.DqcError("query %d (%s x%.2f) failed with code %d" % (i, queries[i].config, queries[i].scale,
                                                                            rec.error))
     return recs, srv
+
+
+def perConfig(queries, recs):
+    """Per config: queries, mean scale, mean and total service time
+    (admission to completion, ms) and retries."""
+    out = OrderedDict()
+    for c in CONFIG_PLANS:
+        idx = [i for i, q in enumerate(queries) if q.config == c]
+        if not idx:
+            continue
+        svc = [(recs[i].done_ns - recs[i].admit_ns) * 1e-6 for i in idx]
+        out[c] = {"queries": len(idx), "mean_scale": float(np.mean([queries[i].scale for i in idx])),
+                  "mean_service_ms": float(np.mean(svc)), "total_service_ms": float(np.sum(svc)),
+                  "retries": int(sum(recs[i].retries for i in idx))}
+    return out
 
 
 def latencyStats(recs):
@@ -275,7 +293,8 @@ def matchesOracle(config, got, want):
     return sameResult(config, got, want)
 
 
-def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None):
+def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None,
+              max_heavy=1, heavy_mb=64.0):
     """Serving benchmark on resident tables at `scale` x the base sizes.
     Returns a dict of capacity (serial / batched QPS), latency at a fixed
     Poisson rate (load x the serial capacity) and verification results."""
@@ -297,7 +316,7 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     for c in CONFIG_PLANS:
         sample += [i for i in order if queries[i].config == c][:max(1, verify // len(CONFIG_PLANS))]
     out = {"scale": scale, "resident_gb": tables.nbytes() / 1e9, "queries": nqueries, "slots": slots,
-           "arena_gb": arena_gb, "plans": dict(plans.names)}
+           "arena_gb": arena_gb, "max_heavy": max_heavy, "heavy_mb": heavy_mb, "plans": dict(plans.names)}
     # capacity: every query arrives at t = 0
     for q in queries:
         q.arrival_ns = 0
@@ -307,9 +326,19 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
                                        for i in sorted(range(nqueries), key=lambda i: recsS[i].admit_ns -
                                                        recsS[i].done_ns)[:6]))
     out["serial"] = latencyStats(recsS)
+    out["serial_by_config"] = perConfig(queries, recsS)
+    say("serial by config: %s" % json.dumps(out["serial_by_config"]))
+    for i in sorted(range(nqueries), key=lambda i: -recsS[i].retries)[:5]:
+        L = recsS[i].layout
+        say("retried %s x%.2f %d times: ht %s mm %s result %d temp %s split %s" % (
+            queries[i].config, queries[i].scale, recsS[i].retries, list(L.ht_capacity[:L.nhts]),
+            list(L.mm_entries[:L.nhts]), L.result_capacity, list(L.temp_capacity[:L.ntemps]),
+            list(L.split_capacity[:L.nhts])))
     say("serial capacity: %.1f q/s" % out["serial"]["qps"])
-    recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample), log=say)
+    hv = {"max_heavy": max_heavy, "heavy_bytes": int(heavy_mb * (1 << 20))}
+    recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample), log=say, **hv)
     out["batched"] = latencyStats(recsB)
+    out["batched_by_config"] = perConfig(queries, recsB)
     say("batched capacity: %.1f q/s" % out["batched"]["qps"])
     # alone vs mixed, and the CPU oracle, on the sample
     alone_same, oracle_ok = 0, 0
@@ -330,7 +359,7 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     recs, srv = run(tables, plans, queries, 1, arena, log=say)
     srv.destroy()
     out["serial_at_rate"] = latencyStats(recs)
-    recs, srv = run(tables, plans, queries, slots, arena, log=say)
+    recs, srv = run(tables, plans, queries, slots, arena, log=say, **hv)
     srv.destroy()
     out["batched_at_rate"] = latencyStats(recs)
     out["offered_qps"] = rate

@@ -49,7 +49,7 @@ def test_datagen_header_and_exports():
 
 
 def test_serve_structs_match_header():
-    assert ctypes.sizeof(rt.ServeConfig) == 4 + 4 + 8 + 4 + 4 + 4 * 6 == 48
+    assert ctypes.sizeof(rt.ServeConfig) == 4 + 4 + 8 + 4 + 4 + 4 + 8 + 4 * 2 == 48
     assert ctypes.sizeof(rt.ServeRequest) == 8 + 24 + 8 + 8 + 4 + 4 == 56
     assert ctypes.sizeof(rt.ServeRecord) == 8 * 3 + 4 * 4 + 8 + 8 + ctypes.sizeof(rt.Layout) + ctypes.sizeof(rt.Status)
 
