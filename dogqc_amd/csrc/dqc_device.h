@@ -151,17 +151,30 @@ DQC_DEVICE_INLINE bool overflow_raised(const uint32_t* status) {
     return (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ST_HT_OVERFLOW) != 0u;
 }
 
+// Unique join tables come in two tag widths.  64-bit tags: bit 63 set plus
+// the packed key (exact) or a hash.  32-bit tags (keys packing into 32 bits):
+// the packed key + 1, so a table is half the size and more of it stays in
+// L2; the one key whose tag would be 0 (all 32 bits set) lives in an extra
+// slot at index cap.  In both, 0 is the empty slot.
+
 // Insert a tag; duplicates of an existing tag take the next free slot (the
 // unique-join build of the reference also stores duplicate keys,
 // hashing.cuh:81-100).  Starts at slot s with n slots already tried.
 // Returns the slot, or -1 when the table is too full.
-DQC_DEVICE_INLINE int64_t ht_insert_from(uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t s, uint32_t n,
+// Compare-and-swap on a tag word of either width; returns the old value.
+template <typename Tag>
+DQC_DEVICE_INLINE Tag cas_tag(Tag* p, Tag expected, Tag desired) {
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expected;
+}
+
+template <typename Tag>
+DQC_DEVICE_INLINE int64_t ht_insert_from(Tag* tags, uint32_t cap, Tag tag, uint32_t s, uint32_t n,
                                          const uint32_t* status) {
     for (; n < cap && n < MAX_PROBE; ++n) {
         if ((n & 31u) == 31u && overflow_raised(status)) return -1;
-        uint64_t prev = atomicCAS((unsigned long long*)&tags[s], (unsigned long long)TAG_EMPTY,
-                                  (unsigned long long)tag);
-        if (prev == TAG_EMPTY) return (int64_t)s;
+        if (cas_tag(&tags[s], (Tag)TAG_EMPTY, tag) == (Tag)TAG_EMPTY) return (int64_t)s;
         s = next_slot(s, cap);
     }
     return -1;
@@ -169,7 +182,7 @@ DQC_DEVICE_INLINE int64_t ht_insert_from(uint64_t* tags, uint32_t cap, uint64_t 
 
 DQC_DEVICE_INLINE int64_t ht_insert(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h,
                                     const uint32_t* status) {
-    return ht_insert_from(tags, cap, tag, slot_of(h, cap), 0u, status);
+    return ht_insert_from<uint64_t>(tags, cap, tag, slot_of(h, cap), 0u, status);
 }
 
 // Find the next slot holding `tag`, starting at *cursor (initialise the
@@ -177,34 +190,36 @@ DQC_DEVICE_INLINE int64_t ht_insert(uint64_t* tags, uint32_t cap, uint64_t tag, 
 // full cycle.  Used by join probes, which run after the build kernel ended.
 DQC_DEVICE_INLINE uint32_t probe_start(uint64_t h, uint32_t cap) { return slot_of(h, cap); }
 
-DQC_DEVICE_INLINE int64_t ht_probe(const uint64_t* tags, uint32_t cap, uint64_t tag, uint32_t& cursor,
-                                   uint32_t& visited) {
+template <typename Tag>
+DQC_DEVICE_INLINE int64_t ht_probe(const Tag* tags, uint32_t cap, Tag tag, uint32_t& cursor, uint32_t& visited) {
     while (visited < cap && visited < MAX_PROBE) {
         uint32_t s = cursor;
-        uint64_t t = tags[s];
+        const Tag t = tags[s];
         cursor = next_slot(s, cap);
         ++visited;
         if (t == tag) return (int64_t)s;
-        if (t == TAG_EMPTY) { visited = cap; return -1; }
+        if (t == (Tag)TAG_EMPTY) { visited = cap; return -1; }
     }
     return -1;
 }
 
 // The next PROBE_WINDOW slots s, s+1, ... of a unique join table, read with
-// two 16-byte loads so one memory round trip resolves most probes, hits and
-// misses alike.  Join tag buffers carry PROBE_WINDOW-1 padding slots so the
+// 16-byte loads (two for 64-bit tags, one for 32-bit tags) so one memory
+// round trip resolves most probes, hits and misses alike.  Join tag buffers
+// carry PROBE_WINDOW padding slots (the 32-bit extra slot included) so the
 // window never leaves the allocation; slots at or past cap are ignored.
 constexpr uint32_t PROBE_WINDOW = 4;
 
+template <typename Tag>
 struct probe_win {
-    uint64_t t[PROBE_WINDOW];
+    Tag t[PROBE_WINDOW];
 };
 
-DQC_DEVICE_INLINE probe_win probe_window(const uint64_t* tags, uint32_t s) {
+DQC_DEVICE_INLINE probe_win<uint64_t> probe_window(const uint64_t* tags, uint32_t s) {
     typedef uint64_t u64x2 __attribute__((ext_vector_type(2), aligned(8)));
     const u64x2* q = (const u64x2*)(tags + s);
     const u64x2 a = q[0], b = q[1];
-    probe_win w;
+    probe_win<uint64_t> w;
     w.t[0] = a.x;
     w.t[1] = a.y;
     w.t[2] = b.x;
@@ -212,10 +227,22 @@ DQC_DEVICE_INLINE probe_win probe_window(const uint64_t* tags, uint32_t s) {
     return w;
 }
 
+DQC_DEVICE_INLINE probe_win<uint32_t> probe_window(const uint32_t* tags, uint32_t s) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4), aligned(4)));
+    const u32x4 a = *(const u32x4*)(tags + s);
+    probe_win<uint32_t> w;
+    w.t[0] = a.x;
+    w.t[1] = a.y;
+    w.t[2] = a.z;
+    w.t[3] = a.w;
+    return w;
+}
+
 // Resolve a probe from its window: the slot holding `tag` (>= 0), -1 when an
 // empty slot comes first, -2 when every window slot holds another tag; then
 // the probe continues with ht_probe at `cursor` with `visited` slots tried.
-DQC_DEVICE_INLINE int64_t probe_in_window(const probe_win& w, uint32_t s, uint32_t cap, uint64_t tag,
+template <typename Tag>
+DQC_DEVICE_INLINE int64_t probe_in_window(const probe_win<Tag>& w, uint32_t s, uint32_t cap, Tag tag,
                                           uint32_t& cursor, uint32_t& visited) {
     int64_t r = -2;
     uint32_t n = 0;
@@ -225,7 +252,7 @@ DQC_DEVICE_INLINE int64_t probe_in_window(const probe_win& w, uint32_t s, uint32
             ++n;
             if (w.t[i] == tag)
                 r = (int64_t)(s + i);
-            else if (w.t[i] == TAG_EMPTY)
+            else if (w.t[i] == (Tag)TAG_EMPTY)
                 r = -1;
         }
     }

@@ -89,12 +89,28 @@ class EquiJoinTranslator(BinaryTranslator):
         self.stored = list(stored.values())
         self.keyReconstruct = [b for b in self.buildKeys if b.id not in stored and
                                any(b.id == x.id for x in above + condBuild)]
+        # unique tables whose packed keys fit 32 bits use 32-bit tags
+        self.tag32 = (self.exact and not a.multimatch and
+                      sum(PACK_BITS[b.dataType] for b in self.buildKeys) <= 32)
+        self.tagType = "uint32_t" if self.tag32 else "uint64_t"
 
     def _hashCode(self, pipe, keys, tagVar, hashVar, tagIsReg=False):
         """Emit the tag and hash of a key tuple.  With tagIsReg the tag goes
         into the per-row register ``tagVar`` (for two-phase lookups)."""
-        tagDecl = tagVar if tagIsReg else "const uint64_t " + tagVar
-        if self.exact:
+        tagDecl = tagVar if tagIsReg else "const %s %s" % (self.tagType, tagVar)
+        if self.tag32:
+            # packed key + 1; the key with all 32 bits set gets tag 0 (extra slot)
+            parts = []
+            shift = 0
+            for k in keys:
+                w = PACK_BITS[k.dataType]
+                mask = "0xffffffffu" if w == 32 else "0xffu"
+                parts.append("(((uint32_t)(%s) & %s) << %d)" % (pipe.access(k), mask, shift))
+                shift += w
+            pipe.emit("const uint32_t %s_key = %s;" % (hashVar, " | ".join(parts)))
+            pipe.emit("%s = %s_key + 1u;" % (tagDecl, hashVar))
+            pipe.emit("const uint64_t %s = dqc::mix64(dqc::TAG_TOP | (uint64_t)%s_key);" % (hashVar, hashVar))
+        elif self.exact:
             parts = ["dqc::TAG_TOP"]
             shift = 0
             for k in keys:
@@ -135,11 +151,13 @@ class EquiJoinTranslator(BinaryTranslator):
         self.storedBuffers = []
         if kind == "unique":
             prog.hostVars["mment%d" % idx] = "L.mment[%d] = 0;" % idx
-            # padded by PROBE_WINDOW - 1 slots for the probe's window loads
-            tags = prog.buffer("jht%d_tags" % J, CType.UINT64, "L.htcap[%d] + 3" % idx, "0x0ull")
-            ftags = pipe.param("jht%d" % J, "uint64_t*", "B_" + tags)
+            # padded by PROBE_WINDOW slots for the probe's window loads (with
+            # 32-bit tags the first of them is the extra slot)
+            tt = CType.UINT if self.tag32 else CType.UINT64
+            tags = prog.buffer("jht%d_tags" % J, tt, "L.htcap[%d] + 4" % idx, "0x0ull")
+            ftags = pipe.param("jht%d" % J, "%s*" % self.tagType, "B_" + tags)
             fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
-            count = "L.htcap[%d]" % idx
+            count = "L.htcap[%d] + 1" % idx
         else:
             prog.hostVars["mment%d" % idx] = (
                 "L.mment[%d] = t->mm_entries[%d] > 0 ? t->mm_entries[%d] : std::max<int64_t>(64, %s * 2);\n"
@@ -159,25 +177,36 @@ class EquiJoinTranslator(BinaryTranslator):
             fields.append(pipe.param("jht%d_p%d" % (J, k), "%s*" % ct, "B_" + b))
         if kind == "unique":
             # first CAS of every row of the thread first, collisions after
-            jtag = pipe.reg("jtag%d" % J, "uint64_t")
+            tt = self.tagType
+            jtag = pipe.reg("jtag%d" % J, tt)
             jc = pipe.reg("jc%d" % J, "uint32_t")
-            jprev = pipe.reg("jprev%d" % J, "uint64_t")
+            jprev = pipe.reg("jprev%d" % J, tt)
             with pipe.rowLoop():
                 pipe.code.open("if (active[u])")
                 self._hashCode(pipe, self.buildKeys, jtag, "jh%d" % J, tagIsReg=True)
-                pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jc, J, fcap))
-                pipe.emit("%s = atomicCAS((unsigned long long*)&%s[%s], 0ull, (unsigned long long)%s);"
-                          % (jprev, ftags, jc, jtag))
+                if self.tag32:
+                    # tag 0: the extra slot at index cap (duplicates of that key are dropped)
+                    pipe.emit("%s = %s != 0u ? dqc::slot_of(jh%d, %s) : %s;" % (jc, jtag, J, fcap, fcap))
+                    pipe.emit("%s = dqc::cas_tag(&%s[%s], (%s)0, %s != 0u ? %s : 1u);"
+                              % (jprev, ftags, jc, tt, jtag, jtag))
+                else:
+                    pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jc, J, fcap))
+                    pipe.emit("%s = dqc::cas_tag(&%s[%s], (%s)0, %s);" % (jprev, ftags, jc, tt, jtag))
                 pipe.code.close()
             with pipe.rowLoop():
                 pipe.code.open("if (active[u])")
-                pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : dqc::ht_insert_from(%s, %s, %s, "
-                          "dqc::next_slot(%s, %s), 1u, &st->flags);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
+                if self.tag32:
+                    pipe.emit("const int64_t js = %s == 0u ? (int64_t)%s : %s == 0u ? (int64_t)-2 : "
+                              "dqc::ht_insert_from<uint32_t>(%s, %s, %s, dqc::next_slot(%s, %s), 1u, &st->flags);"
+                              % (jprev, jc, jtag, ftags, fcap, jtag, jc, fcap))
+                else:
+                    pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : dqc::ht_insert_from<uint64_t>(%s, %s, %s, "
+                              "dqc::next_slot(%s, %s), 1u, &st->flags);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
                 pipe.code.open("if (js >= 0)")
                 for f, att in zip(fields, self.stored):
                     pipe.emit("%s[js] = %s;" % (f, pipe.access(att)))
                 pipe.code.close()
-                pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
+                pipe.emit("else if (js == -1) { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
                           "atomicOr(&st->ht_overflow_mask, %du); }" % (1 << idx))
                 pipe.code.close()
             return
@@ -333,7 +362,8 @@ class EquiJoinTranslator(BinaryTranslator):
         condProbe = list(getattr(a, "conditionProbeAttributes", {}).values())
         pipe.need(condProbe)
         if self.ht["kind"] == "unique":
-            ftags = pipe.param("jht%d" % J, "const uint64_t*", "B_jht%d_tags" % J)
+            tt = self.tagType
+            ftags = pipe.param("jht%d" % J, "const %s*" % tt, "B_jht%d_tags" % J)
             fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
             fields = self._storedFields(pipe)
             slotName = "jslot%d" % J
@@ -341,9 +371,9 @@ class EquiJoinTranslator(BinaryTranslator):
             pipe.indexRegs[slotName] = "int64_t"
             self._bindBuild(pipe, fields, slotName)
             needCheck = (not self.exact) or a.conditions is not None
-            jtag = pipe.reg("jtag%d" % J, "uint64_t")
+            jtag = pipe.reg("jtag%d" % J, tt)
             jc = pipe.reg("jc%d" % J, "uint32_t")
-            jw = pipe.reg("jw%d" % J, "dqc::probe_win")
+            jw = pipe.reg("jw%d" % J, "dqc::probe_win<%s>" % tt)
             if not needCheck:
                 # the probe window of every row of the thread first
                 # (independent loads), then the comparisons
@@ -361,15 +391,29 @@ class EquiJoinTranslator(BinaryTranslator):
                 if needCheck:
                     self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True)
                     pipe.emit("uint32_t jcur = dqc::probe_start(jh%d, %s), jv = 0;" % (J, fcap))
-                    pipe.code.open("for (;;)")
-                    pipe.emit("%s = dqc::ht_probe(%s, %s, %s, jcur, jv);" % (slot, ftags, fcap, jtag))
+                    if self.tag32:
+                        # tag 0: only the extra slot can hold the key
+                        pipe.code.open("if (%s == 0u)" % jtag)
+                        pipe.emit("%s = %s[%s] != 0u ? (int64_t)%s : -1;" % (slot, ftags, fcap, fcap))
+                        pipe.emit("if (%s >= 0 && !(%s)) %s = -1;"
+                                  % (slot, self._matchCond(ctxt, pipe, fields, slotName), slot))
+                        pipe.code.close()
+                        pipe.code.open("else for (;;)")
+                    else:
+                        pipe.code.open("for (;;)")
+                    pipe.emit("%s = dqc::ht_probe<%s>(%s, %s, %s, jcur, jv);" % (slot, tt, ftags, fcap, jtag))
                     pipe.emit("if (%s < 0) break;" % slot)
                     pipe.emit("if (%s) break;" % self._matchCond(ctxt, pipe, fields, slotName))
                     pipe.code.close()
                 else:
                     pipe.emit("uint32_t jcur = 0, jv = 0;")
-                    pipe.emit("%s = dqc::probe_in_window(%s, %s, %s, %s, jcur, jv);" % (slot, jw, jc, fcap, jtag))
-                    pipe.emit("if (%s == -2) %s = dqc::ht_probe(%s, %s, %s, jcur, jv);" % (slot, slot, ftags, fcap, jtag))
+                    pipe.emit("%s = dqc::probe_in_window<%s>(%s, %s, %s, %s, jcur, jv);"
+                              % (slot, tt, jw, jc, fcap, jtag))
+                    pipe.emit("if (%s == -2) %s = dqc::ht_probe<%s>(%s, %s, %s, jcur, jv);"
+                              % (slot, slot, tt, ftags, fcap, jtag))
+                    if self.tag32:
+                        pipe.emit("if (%s == 0u) %s = %s[%s] != 0u ? (int64_t)%s : -1;"
+                                  % (jtag, slot, ftags, fcap, fcap))
                 if a.joinType == Join.ANTI:
                     pipe.emit("active[u] = %s < 0;" % slot)
                 else:
@@ -384,7 +428,7 @@ class EquiJoinTranslator(BinaryTranslator):
                 # a group-by on this join's key can accumulate at the join slot
                 pipe.probedJoins.append({"join": self, "slot": slotName, "htIndex": idx, "opId": J,
                                          "loopOps": pipe.loopOps,
-                                         "buildKeyIds": [b.id for b in self.buildKeys],
+                                         "buildKeyIds": [b.id for b in self.buildKeys], "tag32": self.tag32,
                                          "probeKeyIds": [p.id for p in self.probeKeys]})
             self.parent.consume(ctxt)
             return

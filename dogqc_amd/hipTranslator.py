@@ -687,7 +687,7 @@ class HipCompiler(object):
         elif isinstance(algebraExpr, ra.EquiJoin):
             op = EquiJoinTranslator(algebraExpr, child1, child2)
         elif isinstance(algebraExpr, ra.CrossJoin):
-            raise NotImplementedError("cross joins are not supported by the HIP backend yet")
+            op = CrossJoinTranslator(algebraExpr, child1, child2)
         elif isinstance(algebraExpr, ra.Aggregation):
             op = AggregationTranslator(algebraExpr, child1)
         elif isinstance(algebraExpr, ra.Projection):
@@ -757,6 +757,11 @@ def pipelineShape(translator):
         p = t.parent
         if isinstance(p, LaneRefill):
             loops = True
+        elif isinstance(p, CrossJoinTranslator):
+            if p.leftChild is t:
+                break
+            loops = True
+            beforeJoin = False
         elif isinstance(p, EquiJoinTranslator):
             if p.leftChild is t:
                 break
@@ -971,14 +976,23 @@ class MaterializeTranslator(UnaryTranslator):
 
     def consume(self, ctxt):
         a = self.algExpr
+        isResult = a.matType == MaterializationType.RESULT
+        materializeRows(ctxt, list(a.outRelation.values()), isResult, None if isResult else a.table["name"],
+                        ctxt.program.estimate(a), a.opId)
+
+
+def materializeRows(ctxt, attrs, isResult, tableName, est, opId):
+    """Append the active rows' `attrs` to the result (isResult) or to the
+    temp table `tableName` (see MaterializeTranslator); returns the temp
+    table's record."""
+    if True:
         pipe = ctxt.pipe
         prog = ctxt.program
-        attrs = list(a.outRelation.values())
         pipe.need(attrs)
-        est = prog.estimate(a)
-        isResult = a.matType == MaterializationType.RESULT
+        tmp = None
+        a = None
         if isResult:
-            pipe.emit("// materialize result (opId %d)" % a.opId)
+            pipe.emit("// materialize result (opId %d)" % opId)
             prog.hostVars["rcap"] = ("L.rcap = t->result_capacity > 0 ? t->result_capacity : "
                                      "std::max<int64_t>(%s, 1);" % est)
             cap = pipe.param("rcap", "uint64_t", "L.rcap")
@@ -988,9 +1002,9 @@ class MaterializeTranslator(UnaryTranslator):
             # temp table: same append path into workspace columns; string
             # attributes keep their (start, end) pointers into the resident
             # char heap
-            tmp = prog.newTemp(a.table["name"], est)
+            tmp = prog.newTemp(tableName, est)
             ti = tmp["index"]
-            pipe.emit("// materialize temp table %s (opId %d)" % (a.table["name"], a.opId))
+            pipe.emit("// materialize temp table %s (opId %d)" % (tableName, opId))
             prog.hostVars["tcap%d" % ti] = ("L.tcap[%d] = std::max<int64_t>(1, t->temp_capacity[%d] > 0 ? "
                                            "t->temp_capacity[%d] : %s);" % (ti, ti, ti, est))
             cap = pipe.param("tcap%d" % ti, "uint64_t", "L.tcap[%d]" % ti)
@@ -1027,7 +1041,7 @@ class MaterializeTranslator(UnaryTranslator):
                 c["slot"] = base + i
             tmp["columns"] = public
         NW = pipe.bs // WAVE
-        SW = max(64, min(1024, (self.STAGE_BYTES_PER_WAVE // max(1, rowBytes)) // 64 * 64))
+        SW = max(64, min(1024, (MaterializeTranslator.STAGE_BYTES_PER_WAVE // max(1, rowBytes)) // 64 * 64))
         for i, c in enumerate(cols):
             pipe.staticLds.append("__shared__ %s stg%d[%d][%d];" % (c["ctype"], i, NW, SW))
         wcnt = pipe.reg("stg_n", "uint32_t", "0u", perRow=False)
@@ -1067,6 +1081,74 @@ class MaterializeTranslator(UnaryTranslator):
         pipe.epilogue.open("if (%s > 0u)" % wcnt)
         flush(pipe.epilogue)
         pipe.epilogue.close()
+        return tmp
+
+
+class CrossJoinTranslator(BinaryTranslator):
+    """Nested-loop join (cudaTranslator.py:369-397, relationalAlgebra.py
+    crossjoin): the left input is materialised into a workspace table, and
+    every row of the right pipeline walks all of its rows, evaluating the
+    join condition per pair.  The walk is wavefront-uniform: all lanes read
+    the same inner row in a trip, one broadcast load per value.  Meant for
+    the small inner sides of TPC-H Q7/Q11/Q22 (single aggregates, nation
+    pairs)."""
+
+    def __init__(self, algExpr, leftChild, rightChild):
+        BinaryTranslator.__init__(self, algExpr, leftChild, rightChild)
+        self.consumeCall = 0
+
+    def produce(self, ctxt):
+        self.leftChild.produce(ctxt)
+        self.rightChild.produce(ctxt)
+
+    def _innerAttrs(self):
+        a = self.algExpr
+        used = set(a.outRelation) | set(getattr(a, "conditionAttributes", {}))
+        return [att for id, att in a.leftChild.outRelation.items() if id in used]
+
+    def consume(self, ctxt):
+        self.consumeCall += 1
+        a = self.algExpr
+        if self.consumeCall % 2 == 1:
+            attrs = self._innerAttrs()
+            ctxt.pipe.emit("// nested-loop join: materialise the inner side (opId %d)" % a.opId)
+            self.inner = materializeRows(ctxt, attrs, False, "inner%d" % a.opId,
+                                         ctxt.program.estimate(a.leftChild), a.opId)
+            return
+        pipe = ctxt.pipe
+        J = a.opId
+        tmp = self.inner
+        ti = tmp["index"]
+        pipe.emit("// nested-loop join (opId %d)" % J)
+        fcap = pipe.param("xcap%d" % J, "uint64_t", "L.tcap[%d]" % ti)
+        pos = pipe.reg("xpos%d" % J, "uint32_t", "0")
+        pipe.indexRegs["xpos%d" % J] = "uint32_t"
+        byName = {c["name"]: c for c in tmp["columns"]}
+        for att in self._innerAttrs():
+            c = byName[att.name]
+            f = pipe.param("x%d_%d" % (J, c["pos"]), "const %s*" % c["ctype"], "B_%s" % c["buffer"])
+            pipe.setLazy(att, "%s[%s]" % (f, pos), ["xpos%d" % J], c["ctype"])
+            if c["char_col"] is not None:
+                ctxt.program.charSource[att.id] = c["char_col"]
+        condRight = [att for id, att in getattr(a, "conditionAttributes", {}).items()
+                     if id in a.rightChild.outRelation]
+        pipe.need(condRight)
+        pipe.emit("const uint64_t xn%d = std::min<uint64_t>(st->temp_count[%d], %s);" % (J, ti, fcap))
+        pipe.emit("const int xact%d = active[u];" % J)
+        pipe.loopOps += 1
+        pipe.code.open("for (uint64_t xj = 0; xj < xn%d; ++xj)" % J)
+        pipe.emit("%s = (uint32_t)xj;" % pos)
+        pipe.emit("active[u] = xact%d;" % J)
+        pipe.need([att for att in self._innerAttrs() if att.id in getattr(a, "conditionAttributes", {})])
+        cond = ctxt.exprEmitter().emit(a.condition)
+        pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % cond)
+        ctxt.innerLoopCount += 1
+        pipe.innerDepth += 1
+        self.parent.consume(ctxt)
+        pipe.innerDepth -= 1
+        ctxt.innerLoopCount -= 1
+        pipe.code.close()
+        pipe.emit("active[u] = 0;")
 
 
 class AggregationTranslator(UnaryTranslator):
@@ -1292,7 +1374,7 @@ class AggregationTranslator(UnaryTranslator):
         a = self.algExpr
         pj = self.fused
         pipe.annotations.append("GJ%d" % pj["opId"])
-        cap = "L.htcap[%d]" % pj["htIndex"]
+        cap = "L.htcap[%d]%s" % (pj["htIndex"], " + 1" if pj["tag32"] else "")
         self.accBuffers = []
         for acc in self.accs:
             b = prog.buffer("agg%d_%d" % (a.opId, acc["i"]), acc["ctype"], cap,
@@ -1396,8 +1478,8 @@ class AggregationTranslator(UnaryTranslator):
         prog = ctxt.program
         name = "krnl_aggregation%d" % a.opId
         rows = "L.htcap[%d]" % (self.fused["htIndex"] if self.fused else self.ht["index"])
-        if self.fused is None and self.doGroup and self.exact64:
-            rows += " + 1"
+        if (self.fused is None and self.doGroup and self.exact64) or (self.fused is not None and self.fused["tag32"]):
+            rows += " + 1"     # the extra slot
         pipe = prog.newPipeline(name, {"rows": rows, "tidName": "agg%d" % a.opId}, rowsPerThread(self))
         ctxt.pipe = pipe
         tid = pipe.tid + "[u]"
@@ -1411,13 +1493,20 @@ class AggregationTranslator(UnaryTranslator):
             J = pj["opId"]
             join = pj["join"]
             fhit = pipe.param("agg%d_hit" % a.opId, "const int8_t*", "B_" + self.touched)
-            ftags = pipe.param("jht%d" % J, "const uint64_t*", "B_jht%d_tags" % J)
+            tt = "uint32_t" if pj["tag32"] else "uint64_t"
+            ftags = pipe.param("jht%d" % J, "const %s*" % tt, "B_jht%d_tags" % J)
             gtag = pipe.reg("gtag", CType.UINT64, "0")
             keyIds = pj["buildKeyIds"] if sorted(g.id for g in groups) == sorted(pj["buildKeyIds"]) \
                 else pj["probeKeyIds"]
             with pipe.rowLoop():
                 pipe.emit("if (active[u]) active[u] = %s[%s] != 0;" % (fhit, tid))
-                pipe.emit("if (active[u]) %s = %s[%s];" % (gtag, ftags, tid))
+                if pj["tag32"]:
+                    # the packed key: tag - 1, all bits set in the extra slot
+                    fjcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % pj["htIndex"])
+                    pipe.emit("if (active[u]) %s = %s == (int64_t)%s ? 0xffffffffull : (uint64_t)(%s[%s] - 1u);"
+                              % (gtag, tid, fjcap, ftags, tid))
+                else:
+                    pipe.emit("if (active[u]) %s = %s[%s];" % (gtag, ftags, tid))
                 for g in groups:
                     k = keyIds.index(g.id)
                     shift = sum(PACK_BITS[b.dataType] for b in join.buildKeys[:k])

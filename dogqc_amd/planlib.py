@@ -172,6 +172,42 @@ def _featurePlan(name, alg):
         p = alg.aggregation(["t_bq"], [(Reduction.SUM, "t_f", "sf"), (Reduction.COUNT, "", "c")],
                             alg.map("t_bq", scal.DivExpr(_A("t_big"), _C("1073741824", Type.INT64)), alg.scan("t")))
         return p, {3: {"numgroups": 2100}}
+    if name == "f_join_neg":
+        # unique join (32-bit tags) on a key that includes -1, the key stored
+        # in the table's extra slot; grouped by the build key (group-join)
+        p = alg.aggregation(["uk"], [(Reduction.SUM, "t_val", "s"), (Reduction.COUNT, "", "c")],
+                            alg.join(("uk", "t_grp"),
+                                     alg.map("uk", scal.SubExpr(scal.DivExpr(_A("u_id"), _C("3", Type.INT)),
+                                                                _C("1", Type.INT)), alg.scan("u")),
+                                     alg.scan("t")))
+        return p, {4: {"multimatch": False}}
+    if name == "f_anti_neg":
+        p = alg.aggregation(["t_cat"], [(Reduction.COUNT, "", "c")],
+                            alg.antijoin(("uk", "t_grp"), None,
+                                         alg.map("uk", scal.SubExpr(scal.DivExpr(_A("u_id"), _C("3", Type.INT)),
+                                                                    _C("1", Type.INT)), alg.scan("u")),
+                                         alg.scan("t")))
+        return p, {4: {"multimatch": False}}
+    if name == "f_cross":
+        # TPC-H Q11 shape (query/plans/tpch11.py): groups whose sum exceeds
+        # a fraction of the total, via a nested-loop join with a one-row side
+        p = alg.projection(["t_grp", "sv"],
+                           alg.crossjoin(scal.LargerExpr(_A("sv"), _A("lim")),
+                                         alg.map("lim", scal.MulExpr(_A("tot"), _C("0.02", Type.DOUBLE)),
+                                                 alg.aggregation([], [(Reduction.SUM, "t_f", "tot")],
+                                                                 alg.scan("t"))),
+                                         alg.aggregation(["t_grp"], [(Reduction.SUM, "t_f", "sv")],
+                                                         alg.scan("t"))))
+        return p, {6: {"numgroups": 128}}
+    if name == "f_cross_pairs":
+        # a ten-row inner side, string condition, counted per probe category
+        p = alg.aggregation(["t_cat"], [(Reduction.COUNT, "", "c"), (Reduction.SUM, "u_w", "sw")],
+                            alg.crossjoin(scal.AndExpr(scal.LargerExpr(_A("u_w"), _A("t_f")),
+                                                       scal.EqualsExpr(_A("u_name"), _C("echo", Type.STRING))),
+                                          alg.selection(scal.SmallerExpr(_A("u_id"), _C("90", Type.INT)),
+                                                        alg.scan("u")),
+                                          alg.scan("t")))
+        return p, {2: {"selectivity": 0.05}}
     if name == "f_outer":
         # TPC-H Q13 shape (query/plans/tpch13.py): probe rows without a
         # match count 0 through COUNT(build attribute)
@@ -217,7 +253,8 @@ def _featurePlan(name, alg):
 
 FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
                  "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
-                 "f_groupby_int64", "f_outer", "f_outer_project"]
+                 "f_groupby_int64", "f_outer", "f_outer_project", "f_join_neg", "f_anti_neg",
+                 "f_cross", "f_cross_pairs"]
 
 # name -> (workload or "feature", options)
 PLANS = OrderedDict()

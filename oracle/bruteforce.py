@@ -98,6 +98,15 @@ class BruteForce(object):
                         row[("isnull", id)] = True
                     out.append(row)
             return out
+        if isinstance(node, ra.CrossJoin):
+            out = []
+            for p in self.rows(node.rightChild):
+                for b in self.rows(node.leftChild):
+                    pair = dict(b)
+                    pair.update(p)
+                    if self.value(node.condition, pair):
+                        out.append(pair)
+            return out
         if isinstance(node, ra.Aggregation):
             rows = self.rows(node.child)
             groups = OrderedDict()

@@ -375,6 +375,23 @@ class Evaluator(object):
             return out, len(pi) + len(miss)
         raise NotImplementedError(str(node.joinType))
 
+    def _CrossJoin(self, node):
+        """Every (left, right) pair that satisfies the condition
+        (cudaTranslator.py:369-397)."""
+        L, nl = self.node(node.leftChild)
+        R, nr = self.node(node.rightChild)
+        li = np.tile(np.arange(nl, dtype=np.int64), nr)
+        ri = np.repeat(np.arange(nr, dtype=np.int64), nl)
+        pairs = {k: v[li] for k, v in L.items()}
+        pairs.update({k: v[ri] for k, v in R.items()})
+        n = nl * nr
+        if n:
+            m, _ = self.expr(node.condition, pairs, n)
+            keep = np.broadcast_to(np.asarray(m) != 0, (n,))
+            pairs = {k: v[keep] for k, v in pairs.items()}
+            n = int(keep.sum())
+        return pairs, n
+
     def _Aggregation(self, node):
         rel, n = self.node(node.child)
         groups = list(node.groupAttributes.values())
