@@ -1,7 +1,8 @@
 """Ahead-of-time build: the runtime library and every named plan module.
 
 ``buildAll()`` compiles ``csrc/dqc_runtime.hip`` and ``csrc/dqc_serve.hip``
-into ``libdqcrt.so`` and
+into ``libdqcrt.so``, ``csrc/dqc_gen.hip`` into ``libdqcgen.so``,
+``csrc/dqc_import.cpp`` into ``libdqcimport.so`` (host code) and
 every plan of ``planlib.PLANS`` into ``_plans/`` (in parallel), always from
 source, for gfx950.
 """
@@ -42,6 +43,23 @@ def buildGen():
         raise RuntimeError("building libdqcgen.so failed:\n" + proc.stderr[-6000:])
     os.replace(tmp, GEN_SO)
     return GEN_SO
+
+
+IMPORT_SRC = os.path.join(PKG, "csrc", "dqc_import.cpp")
+IMPORT_SO = os.path.join(PKG, "libdqcimport.so")
+
+
+def buildImport():
+    """libdqcimport.so: the .tbl importer (host code only, built with g++;
+    -ffp-contract=off keeps the decimal parser's float steps unfused)."""
+    tmp = IMPORT_SO + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-Wall",
+           "-I", plancache.INCLUDE, IMPORT_SRC, "-o", tmp]
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("building libdqcimport.so failed:\n" + proc.stderr[-6000:])
+    os.replace(tmp, IMPORT_SO)
+    return IMPORT_SO
 
 
 def buildPlans(names=None, jobs=None, force=False):
@@ -87,11 +105,12 @@ def buildAll(verbose=True):
     t0 = time.time()
     buildRuntime()
     buildGen()
+    buildImport()
     paths = buildPlans(force=True)
     variants = buildVariants()
     cleanStale(list(paths.values()) + variants)
     if verbose:
-        print("built libdqcrt.so, libdqcgen.so and %d plan modules in %.1f s" % (len(paths) + len(variants),
+        print("built libdqcrt.so, libdqcgen.so, libdqcimport.so and %d plan modules in %.1f s" % (len(paths) + len(variants),
                                                                                  time.time() - t0),
               file=sys.stderr)
     return paths

@@ -11,11 +11,14 @@ The reference writer stores only the low 4 header bytes (mappedmalloc.h:26),
 which breaks files of 4 GiB and more; this writer stores all 8, and a file
 written by the reference reads identically (its upper 4 bytes are zero).
 
-``dbAccess`` keeps the reference signature (dbio.py:13-27).  Importing TPC-H
-``.tbl`` text files is not part of this module: tables are written column by
-column with ``writeTable`` (synthetic workloads) and the schema is cached as
-JSON in ``<dbpath>/dbschema.json``.
+``dbAccess`` keeps the reference signature (dbio.py:13-27): given a
+``csvpath`` it imports ``<csvpath>/<table>.tbl`` text tables (dbgen output)
+with the native importer ``libdqcimport.so`` (include/dogqc_amd_import.h)
+unless ``<dbpath>`` already holds them at the same sizes.  Synthetic tables are
+written column by column with ``writeTable``.  The schema is cached as JSON in
+``<dbpath>/dbschema.json`` (the reference pickles it, dbio.py:62-64).
 """
+import ctypes as C
 import json
 import os
 from collections import OrderedDict
@@ -123,12 +126,114 @@ def tableSizeFromFiles(dbpath, table):
     return payloadBytes(columnPath(dbpath, table["name"], attr)) // np.dtype(COLUMN_DTYPE[typ]).itemsize
 
 
+IMPORT_SO = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdqcimport.so")
+
+# include/dogqc_amd_import.h DQC_COL_*
+IMPORT_TYPE = {Type.INT: 0, Type.INT64: 1, Type.FLOAT: 2, Type.DOUBLE: 3, Type.CHAR: 4, Type.DATE: 5,
+               Type.STRING: 6}
+
+
+class ImportReport(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("error_line", C.c_int64), ("error_column", C.c_int32),
+                ("error_kind", C.c_int32), ("char_bytes", C.c_int64 * 64), ("message", C.c_char * 256)]
+
+
+class ImportError_(IOError):
+    """A .tbl file the importer rejects; ``line``/``column``/``kind`` as in
+    dqc_import_report."""
+
+    def __init__(self, report):
+        IOError.__init__(self, report.message.decode(errors="replace"))
+        self.line, self.column, self.kind = report.error_line, report.error_column, report.error_kind
+
+
+_importLib = None
+
+
+def importLib():
+    global _importLib
+    if _importLib is None:
+        if not os.path.exists(IMPORT_SO):
+            raise RuntimeError("libdqcimport.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(IMPORT_SO)
+        L.dqc_tbl_rows.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
+        L.dqc_import_tbl.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_char_p), C.c_int32,
+                                     C.POINTER(ImportReport)]
+        _importLib = L
+    return _importLib
+
+
+def tblRows(path):
+    """Rows of a .tbl file (its lines; Importer.retrieveTableSizes,
+    dbio.py:124-138)."""
+    n = C.c_int64()
+    rc = importLib().dqc_tbl_rows(path.encode(), C.byref(n))
+    if rc != 0:
+        raise IOError("cannot read %s (%d)" % (path, rc))
+    return n.value
+
+
+def importTable(dbpath, table, tblPath, threads=0):
+    """Import one .tbl file into the mmdb column files of ``table`` (a schema
+    dict); returns (rows, {STRING attr: char bytes}).  Replaces the
+    reference's generated importer (dbio.py:172-244)."""
+    attrs = list(table["attributes"].items())
+    types = []
+    for a, t in attrs:
+        if t not in IMPORT_TYPE:
+            raise ValueError("cannot import %s.%s of type %s" % (table["name"], a, t))
+        types.append(IMPORT_TYPE[t])
+    paths = [columnPath(dbpath, table["name"], a).encode() for a, _ in attrs]
+    ctypes_ = (C.c_int32 * len(types))(*types)
+    cpaths = (C.c_char_p * len(paths))(*paths)
+    rep = ImportReport()
+    rc = importLib().dqc_import_tbl(tblPath.encode(), len(types), ctypes_, cpaths, threads, C.byref(rep))
+    if rc != 0:
+        raise ImportError_(rep)
+    chars = {a: int(rep.char_bytes[k]) for k, (a, t) in enumerate(attrs) if t == Type.STRING}
+    return int(rep.rows), chars
+
+
+def _sameTables(cached, sized):
+    if cached is None:
+        return False
+    for name, t in sized.items():
+        c = cached.get(name)
+        if c is None or c["size"] != t["size"] or list(c["attributes"].items()) != list(t["attributes"].items()):
+            return False
+    return True
+
+
 def dbAccess(schema, dbpath, csvpath=None, doReload=False, waitEnter=False):
-    """Open an mmdb directory.  Table sizes come from the cached schema or,
-    failing that, from the column files themselves."""
-    if csvpath is not None and doReload:
-        raise NotImplementedError(".tbl import is not supported; write tables with writeTable()")
+    """Open an mmdb directory (dbio.py:13-27).  With ``csvpath``, table sizes
+    come from the .tbl files, and the tables are imported when the directory
+    does not hold them at those sizes or ``doReload`` is set.  Without it,
+    sizes come from the cached schema or, failing that, the column files."""
+    os.makedirs(dbpath, exist_ok=True)
     cached = Accessor.loadSchema(dbpath)
+    if csvpath is not None:
+        sized = OrderedDict()
+        for name, t in schema.items():
+            if not isinstance(t, dict):
+                continue
+            t = dict(t)
+            t["attributes"] = OrderedDict(t["attributes"])
+            t["size"] = tblRows(os.path.join(csvpath, name + ".tbl"))
+            sized[name] = t
+        if _sameTables(cached, sized) and not doReload:
+            for name, t in sized.items():
+                t["charSizes"] = cached[name].get("charSizes", {})
+        else:
+            if waitEnter:
+                input("Press Enter to OVERWRITE database...")
+            for name, t in sized.items():
+                rows, chars = importTable(dbpath, t, os.path.join(csvpath, name + ".tbl"))
+                t["size"], t["charSizes"] = rows, chars
+        acc = Accessor(dbpath, dict(cached or {}))
+        acc.schema.update(sized)
+        acc.writeSchema()
+        acc.schema = sized
+        return acc
     sized = {}
     for name, t in schema.items():
         if not isinstance(t, dict):
