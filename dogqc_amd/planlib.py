@@ -273,9 +273,18 @@ PLANS["f_join_multi_pushdown"] = ("feature", {"pushdown": True, "base": "f_join_
 PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_project"})
 
 
+# TPC-H Q1-Q22 (dogqc_amd/tpch.py)
+for _q in range(1, 23):
+    PLANS["tpch%d" % _q] = ("tpch", {"query": _q})
+
+
 def build(name, acc):
     """(alg, plan, cfg, options) for a named plan over an accessor."""
     kind, opts = PLANS[name]
+    if kind == "tpch":
+        from dogqc_amd import tpch
+        alg, p, cfg = tpch.QUERIES[opts["query"]](acc)
+        return alg, p, cfg, opts
     if kind == "feature":
         alg = RelationalAlgebra(acc)
         p, cfg = _featurePlan(opts.get("base", name), alg)
@@ -286,6 +295,9 @@ def build(name, acc):
 
 def emptyAccessor(name):
     kind, _ = PLANS[name]
+    if kind == "tpch":
+        from dogqc_amd import tpch
+        return tpch.SchemaAccessor(tpch.schema())
     return featureEmpty() if kind == "feature" else workloads.emptyAccessor(kind)
 
 

@@ -287,3 +287,24 @@ def test_feature_retry_all_buffers():
     t.ht_capacity[0] = 2
     _, retries = check("f_groupby_wide", acc, tuning=t, label="tiny group table")
     assert retries > 0
+
+
+# ------------------------------------------------------------ TPC-H
+
+_TPCH = {}
+
+
+def tpchAcc(sf):
+    if sf not in _TPCH:
+        from dogqc_amd import tpch
+        _TPCH[sf] = MemoryAccessor(tpch.generate(sf, seed=3))
+    return _TPCH[sf]
+
+
+@pytest.mark.parametrize("q", range(1, 23))
+@pytest.mark.parametrize("sf", [0.05, 0.002])
+def test_tpch(q, sf):
+    """TPC-H Q1-Q22 on synthetic TPC-H data (300k and 12k lineitem rows),
+    every operator of the plan API on the GPU."""
+    check("tpch%d" % q, tpchAcc(sf), storeKey=("tpch", sf), label="sf=%g" % sf)
+

@@ -32,6 +32,17 @@ SMALL = {"c0": {"cfg0": 400}, "c1": {"dim": 120, "fact": 900}, "c2": {"build": 1
          "c3": {"sales": 1500, "item": 300, "cust": 40, "nation": 5}}
 
 
+_TPCH = []
+
+
+def tpchAccessor():
+    """Synthetic TPC-H at scale factor 0.001 (6000 lineitem rows)."""
+    if not _TPCH:
+        from dogqc_amd import tpch
+        _TPCH.append(MemoryAccessor(tpch.generate(0.001, seed=3)))
+    return _TPCH[0]
+
+
 def resolved(name, acc):
     alg, p, cfg, _ = planlib.build(name, acc)
     return alg.resolveAlgebraPlan(p, cfg)
@@ -42,6 +53,8 @@ def test_planeval_matches_bruteforce(name):
     kind = planlib.PLANS[name][0]
     if kind == "feature":
         acc = planlib.featureData(160, 40, seed=3)
+    elif kind == "tpch":
+        acc = tpchAccessor()
     else:
         acc = MemoryAccessor(workloads.generate(kind, SMALL[kind], seed=5))
     plan = resolved(name, acc)
