@@ -667,6 +667,7 @@ class HipCompiler(object):
         self.pipe = None
         self.source = None
         self.module = None
+        self.planName = None
 
     def setBuffers(self, positions):
         self.bufferPositions = list(positions)
@@ -717,6 +718,8 @@ class HipCompiler(object):
         if self.source is None:
             raise RuntimeError("gencode() must run before compile()")
         self.module = plancache.compileSource(self.source, filename, debug=self.debug)
+        if not getattr(self, "planName", None):
+            self.planName = filename      # execute() logs to <filename>.log
         return self.module
 
     def execute(self):
@@ -985,103 +988,107 @@ def materializeRows(ctxt, attrs, isResult, tableName, est, opId):
     """Append the active rows' `attrs` to the result (isResult) or to the
     temp table `tableName` (see MaterializeTranslator); returns the temp
     table's record."""
-    if True:
-        pipe = ctxt.pipe
-        prog = ctxt.program
-        pipe.need(attrs)
-        tmp = None
-        a = None
-        if isResult:
-            pipe.emit("// materialize result (opId %d)" % opId)
-            prog.hostVars["rcap"] = ("L.rcap = t->result_capacity > 0 ? t->result_capacity : "
-                                     "std::max<int64_t>(%s, 1);" % est)
-            cap = pipe.param("rcap", "uint64_t", "L.rcap")
-            counter = "st->result_count"
-            bprefix, fprefix = "res", "r"
-        else:
-            # temp table: same append path into workspace columns; string
-            # attributes keep their (start, end) pointers into the resident
-            # char heap
-            tmp = prog.newTemp(tableName, est)
-            ti = tmp["index"]
-            pipe.emit("// materialize temp table %s (opId %d)" % (tableName, opId))
-            prog.hostVars["tcap%d" % ti] = ("L.tcap[%d] = std::max<int64_t>(1, t->temp_capacity[%d] > 0 ? "
-                                           "t->temp_capacity[%d] : %s);" % (ti, ti, ti, est))
-            cap = pipe.param("tcap%d" % ti, "uint64_t", "L.tcap[%d]" % ti)
-            counter = "st->temp_count[%d]" % ti
-            bprefix, fprefix = "tmp%d_" % ti, "w%d_" % ti
-        cols = []
-        rowBytes = 0
-        for i, att in enumerate(attrs):
-            st = pipe.attrs[att.id]
-            ct = CType.STR_OFFS if (isResult and att.dataType == Type.STRING) else st.ctype
-            rowBytes += CType.size.get(ct, 8) if ct != CType.STR_TYPE else 16
-            bname = "%s%d" % (bprefix, i)
-            prog.buffer(bname, ct, "L.rcap" if isResult else "L.tcap[%d]" % ti)
-            f = pipe.param("%s%d" % (fprefix, i), "%s*" % ct, "B_%s" % bname)
-            val = pipe.access(att)
-            null = pipe.nullFlags.get(att.id)
-            if ct == CType.STR_OFFS:
-                ci = prog.charSource[att.id]
-                fc = pipe.param("c%d" % ci, "const char*", "C_%d" % ci)
-                if null:    # a null string is the empty string
-                    val = "(%s ? dqc::str_t{%s, %s} : %s)" % (null, fc, fc, val)
-                val = "dqc::to_string_offset(%s, %s)" % (fc, val)
-            elif null and ct != CType.STR_TYPE:
-                val = "(%s ? (%s)0 : %s)" % (null, ct, val)
-            cols.append({"name": att.name, "type": att.dataType.name, "ctype": ct, "buffer": bname,
-                         "char_col": prog.charSource.get(att.id), "field": f, "val": val})
-        public = [{k: v for k, v in c.items() if k not in ("field", "val")} for c in cols]
-        if isResult:
-            prog.result = {"columns": public}
-        else:
-            base = len(prog.tempColumns())
-            for i, c in enumerate(public):
-                c["pos"] = i
-                c["slot"] = base + i
-            tmp["columns"] = public
-        NW = pipe.bs // WAVE
-        SW = max(64, min(1024, (MaterializeTranslator.STAGE_BYTES_PER_WAVE // max(1, rowBytes)) // 64 * 64))
+    names = [att.name for att in attrs]
+    dup = sorted(set(n for n in names if names.count(n) > 1))
+    if dup:
+        raise ValueError("columns %s of %s %s have the same name; rename them with a map"
+                         % (dup, "the result" if isResult else "temp table", tableName or ""))
+    pipe = ctxt.pipe
+    prog = ctxt.program
+    pipe.need(attrs)
+    tmp = None
+    a = None
+    if isResult:
+        pipe.emit("// materialize result (opId %d)" % opId)
+        prog.hostVars["rcap"] = ("L.rcap = t->result_capacity > 0 ? t->result_capacity : "
+                                 "std::max<int64_t>(%s, 1);" % est)
+        cap = pipe.param("rcap", "uint64_t", "L.rcap")
+        counter = "st->result_count"
+        bprefix, fprefix = "res", "r"
+    else:
+        # temp table: same append path into workspace columns; string
+        # attributes keep their (start, end) pointers into the resident
+        # char heap
+        tmp = prog.newTemp(tableName, est)
+        ti = tmp["index"]
+        pipe.emit("// materialize temp table %s (opId %d)" % (tableName, opId))
+        prog.hostVars["tcap%d" % ti] = ("L.tcap[%d] = std::max<int64_t>(1, t->temp_capacity[%d] > 0 ? "
+                                       "t->temp_capacity[%d] : %s);" % (ti, ti, ti, est))
+        cap = pipe.param("tcap%d" % ti, "uint64_t", "L.tcap[%d]" % ti)
+        counter = "st->temp_count[%d]" % ti
+        bprefix, fprefix = "tmp%d_" % ti, "w%d_" % ti
+    cols = []
+    rowBytes = 0
+    for i, att in enumerate(attrs):
+        st = pipe.attrs[att.id]
+        ct = CType.STR_OFFS if (isResult and att.dataType == Type.STRING) else st.ctype
+        rowBytes += CType.size.get(ct, 8) if ct != CType.STR_TYPE else 16
+        bname = "%s%d" % (bprefix, i)
+        prog.buffer(bname, ct, "L.rcap" if isResult else "L.tcap[%d]" % ti)
+        f = pipe.param("%s%d" % (fprefix, i), "%s*" % ct, "B_%s" % bname)
+        val = pipe.access(att)
+        null = pipe.nullFlags.get(att.id)
+        if ct == CType.STR_OFFS:
+            ci = prog.charSource[att.id]
+            fc = pipe.param("c%d" % ci, "const char*", "C_%d" % ci)
+            if null:    # a null string is the empty string
+                val = "(%s ? dqc::str_t{%s, %s} : %s)" % (null, fc, fc, val)
+            val = "dqc::to_string_offset(%s, %s)" % (fc, val)
+        elif null and ct != CType.STR_TYPE:
+            val = "(%s ? (%s)0 : %s)" % (null, ct, val)
+        cols.append({"name": att.name, "type": att.dataType.name, "ctype": ct, "buffer": bname,
+                     "char_col": prog.charSource.get(att.id), "field": f, "val": val})
+    public = [{k: v for k, v in c.items() if k not in ("field", "val")} for c in cols]
+    if isResult:
+        prog.result = {"columns": public}
+    else:
+        base = len(prog.tempColumns())
+        for i, c in enumerate(public):
+            c["pos"] = i
+            c["slot"] = base + i
+        tmp["columns"] = public
+    NW = pipe.bs // WAVE
+    SW = max(64, min(1024, (MaterializeTranslator.STAGE_BYTES_PER_WAVE // max(1, rowBytes)) // 64 * 64))
+    for i, c in enumerate(cols):
+        pipe.staticLds.append("__shared__ %s stg%d[%d][%d];" % (c["ctype"], i, NW, SW))
+    wcnt = pipe.reg("stg_n", "uint32_t", "0u", perRow=False)
+
+    def flush(code):
+        # one global atomic per wavefront flush, coalesced copy-out by the
+        # lanes currently executing (all 64 unless inside divergent code)
+        code.emit("dqc::wave_lds_sync();")
+        code.emit("const uint64_t fx = dqc::ballot(true);")
+        code.emit("const int fl = dqc::first_lane(fx);")
+        code.emit("unsigned long long wb = 0;")
+        code.emit("if ((int)lane == fl) wb = atomicAdd((unsigned long long*)&%s, (unsigned long long)%s);"
+                  % (counter, wcnt))
+        code.emit("wb = dqc::uniform64(dqc::shfl(wb, fl));")
+        code.open("for (uint32_t i = dqc::lanes_below(fx); i < %s; i += dqc::popc64(fx))" % wcnt)
+        code.emit("const uint64_t wp = wb + i;")
+        code.open("if (wp < %s)" % cap)
         for i, c in enumerate(cols):
-            pipe.staticLds.append("__shared__ %s stg%d[%d][%d];" % (c["ctype"], i, NW, SW))
-        wcnt = pipe.reg("stg_n", "uint32_t", "0u", perRow=False)
+            code.emit("%s[wp] = stg%d[wave][i];" % (c["field"], i))
+        code.close()
+        code.close()
+        code.emit("dqc::wave_lds_sync();")
+        code.emit("%s = 0u;" % wcnt)
 
-        def flush(code):
-            # one global atomic per wavefront flush, coalesced copy-out by the
-            # lanes currently executing (all 64 unless inside divergent code)
-            code.emit("dqc::wave_lds_sync();")
-            code.emit("const uint64_t fx = dqc::ballot(true);")
-            code.emit("const int fl = dqc::first_lane(fx);")
-            code.emit("unsigned long long wb = 0;")
-            code.emit("if ((int)lane == fl) wb = atomicAdd((unsigned long long*)&%s, (unsigned long long)%s);"
-                      % (counter, wcnt))
-            code.emit("wb = dqc::uniform64(dqc::shfl(wb, fl));")
-            code.open("for (uint32_t i = dqc::lanes_below(fx); i < %s; i += dqc::popc64(fx))" % wcnt)
-            code.emit("const uint64_t wp = wb + i;")
-            code.open("if (wp < %s)" % cap)
-            for i, c in enumerate(cols):
-                code.emit("%s[wp] = stg%d[wave][i];" % (c["field"], i))
-            code.close()
-            code.close()
-            code.emit("dqc::wave_lds_sync();")
-            code.emit("%s = 0u;" % wcnt)
-
-        with pipe.rowLoop():
-            pipe.emit("const uint64_t wm = dqc::ballot(active[u]);")
-            pipe.emit("const uint32_t wn = dqc::popc64(wm);")
-            pipe.code.open("if (%s + wn > %du)" % (wcnt, SW))
-            flush(pipe.code)
-            pipe.code.close()
-            pipe.code.open("if (active[u])")
-            pipe.emit("const uint32_t sp = %s + dqc::lanes_below(wm);" % wcnt)
-            for i, c in enumerate(cols):
-                pipe.emit("stg%d[wave][sp] = %s;" % (i, c["val"]))
-            pipe.code.close()
-            pipe.emit("%s += wn;" % wcnt)
-        pipe.epilogue.open("if (%s > 0u)" % wcnt)
-        flush(pipe.epilogue)
-        pipe.epilogue.close()
-        return tmp
+    with pipe.rowLoop():
+        pipe.emit("const uint64_t wm = dqc::ballot(active[u]);")
+        pipe.emit("const uint32_t wn = dqc::popc64(wm);")
+        pipe.code.open("if (%s + wn > %du)" % (wcnt, SW))
+        flush(pipe.code)
+        pipe.code.close()
+        pipe.code.open("if (active[u])")
+        pipe.emit("const uint32_t sp = %s + dqc::lanes_below(wm);" % wcnt)
+        for i, c in enumerate(cols):
+            pipe.emit("stg%d[wave][sp] = %s;" % (i, c["val"]))
+        pipe.code.close()
+        pipe.emit("%s += wn;" % wcnt)
+    pipe.epilogue.open("if (%s > 0u)" % wcnt)
+    flush(pipe.epilogue)
+    pipe.epilogue.close()
+    return tmp
 
 
 class CrossJoinTranslator(BinaryTranslator):

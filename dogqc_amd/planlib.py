@@ -276,6 +276,9 @@ PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_pr
 # TPC-H Q1-Q22 (dogqc_amd/tpch.py)
 for _q in range(1, 23):
     PLANS["tpch%d" % _q] = ("tpch", {"query": _q})
+# Q3 with a lane-refill buffer after the customer selection and a
+# lane-activity profile after the lineitem selection (profiler.py usage)
+PLANS["tpch3_profiled"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7]})
 
 
 def build(name, acc):
@@ -316,6 +319,7 @@ def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=
                threshold=opts.get("threshold"), rowsPerThread=rowsPerThread):
         comp = HipCompiler(alg, "gfx950", decimalRepr)
         comp.setBuffers(opts.get("buffers", []))
+        comp.setProfilers(opts.get("profilers", []))
         tp = alg.translateToCompilerPlan(plan, comp)
         comp.gencode(tp)
     comp.planName = name

@@ -285,9 +285,10 @@ def q7(acc):
                                    scal.EqualsExpr(A("n2.n_name"), _s("GERMANY"))),
                               _and(scal.EqualsExpr(A("n1.n_name"), _s("GERMANY")),
                                    scal.EqualsExpr(A("n2.n_name"), _s("FRANCE")))), all_)
-    plan = alg.aggregation(["n1.n_name", "n2.n_name", "l_year"], [(Reduction.SUM, "volume", "revenue")],
+    named = alg.map("cust_nation", A("n2.n_name"), alg.map("supp_nation", A("n1.n_name"), cross))
+    plan = alg.aggregation(["supp_nation", "cust_nation", "l_year"], [(Reduction.SUM, "volume", "revenue")],
                            alg.map("l_year", scal.ExtractExpr(A("l_shipdate"), scal.ExtractType.YEAR),
-                                   alg.map("volume", _disc(), cross)))
+                                   alg.map("volume", _disc(), named)))
     return alg, plan, q.cfg
 
 
@@ -709,10 +710,13 @@ def generate(sf=0.01, seed=1):
         ("p_retailprice", (Type.FLOAT, retail)),
         ("p_comment", (Type.STRING, _strings(_text(rng, nP, 2, 6))))])
 
-    # four suppliers per part (the specification's spread formula)
+    # four distinct suppliers per part, a quarter of the suppliers apart
+    # (the specification's spread formula without its (partkey-1)/S term,
+    # which repeats a supplier when S is small)
+    step = nS // 4
     psPart = np.repeat(pk, 4)
     psIdx = np.tile(np.arange(4, dtype=np.int64), nP)
-    psSupp = (psPart + psIdx * (nS // 4 + (psPart - 1) // nS)) % nS + 1
+    psSupp = (psPart + psIdx * step) % nS + 1
     nPS = len(psPart)
     T["partsupp"] = OrderedDict([
         ("ps_partkey", (Type.INT, psPart.astype(np.int32))),
@@ -747,7 +751,7 @@ def generate(sf=0.01, seed=1):
     lo = np.repeat(np.arange(nO), nl)
     lnum = (np.arange(nL) - np.repeat(np.cumsum(nl) - nl, nl) + 1).astype(np.int32)
     lpart = rng.integers(1, nP + 1, size=nL)
-    lsupp = (lpart + rng.integers(0, 4, size=nL) * (nS // 4 + (lpart - 1) // nS)) % nS + 1
+    lsupp = (lpart + rng.integers(0, 4, size=nL) * step) % nS + 1
     qty = rng.integers(1, 51, size=nL).astype(np.int32)
     price = (qty * retail[lpart - 1].astype(np.float64)).astype(np.float32)
     disc = (rng.integers(0, 11, size=nL) / 100.0).astype(np.float32)

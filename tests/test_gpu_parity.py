@@ -308,3 +308,36 @@ def test_tpch(q, sf):
     every operator of the plan API on the GPU."""
     check("tpch%d" % q, tpchAcc(sf), storeKey=("tpch", sf), label="sf=%g" % sf)
 
+
+
+def test_reference_driver_flow_with_profiler(tmp_path, monkeypatch):
+    """The reference's driver sequence (query/tpch.py:41-69, profiler.py:
+    27-35): resolve, HipCompiler with profilers and a lane-refill buffer,
+    gencode, compile, execute(); the printed protocol is kept in <name>.log
+    with a <pN> histogram whose iterations add up to the profiled wave trips."""
+    import re
+    from dogqc_amd import tpch
+    from dogqc_amd.hipTranslator import HipCompiler
+    from dogqc_amd.hiplang import CType
+    print("gpu case: driver flow tpch3 with profiler", flush=True)
+    assert planlib.PLANS["tpch3_profiled"][1]["profilers"] == [7]
+    monkeypatch.chdir(tmp_path)
+    acc = tpchAcc(0.002)
+    alg, plan, cfg = tpch.q3(acc)
+    plan = alg.resolveAlgebraPlan(plan, cfg)
+    sel = [n.opId for n in alg.selectionNodes]
+    comp = HipCompiler(algebraContext=alg, smArchitecture="sm_75", decimalRepr=CType.FP64, debug=False)
+    comp.setProfilers(sel[-1:])
+    comp.setBuffers(sel[:1])
+    comp.gencode(alg.translateToCompilerPlan(plan, comp))
+    comp.compile("tpch3_profiled")          # the prebuilt module of this plan
+    out = comp.execute()
+    log = (tmp_path / "tpch3_profiled.log").read_text()
+    assert log == out
+    block = re.search(r"<p%d>\n(.*?)</p%d>" % (sel[-1], sel[-1]), log, re.S).group(1)
+    hist = [int(l.split(",")[1]) for l in block.strip().splitlines()]
+    assert len(hist) == 65 and sum(hist) > 0
+    assert "totalKernelTime" in log and "LR%d" % sel[0] in log and "P%d" % sel[-1] in log
+    want, rplan = oracleFor("tpch3", acc)
+    assertSameResult(comp.result, want, toleranceColumns(rplan), label="driver flow tpch3")
+    assert "Result: %d tuples" % len(want["l_orderkey"]) in log

@@ -333,3 +333,24 @@ def test_c_oracle_agrees_at_full_size():
         assert int(res["count_order"][i]) == r[5]
         np.testing.assert_allclose([res["sum_qty"][i], res["sum_base_price"][i], res["sum_disc_price"][i],
                                     res["sum_charge"][i]], r[:4], rtol=1e-12)
+
+
+@pytest.mark.parametrize("sf", [0.0005, 0.002, 0.05])
+def test_tpch_generator_keys(sf):
+    """The synthetic TPC-H keeps the keys the plans mark unique unique, and
+    the foreign keys inside their referenced tables."""
+    from dogqc_amd import tpch
+    T = tpch.generate(sf, seed=3)
+    col = lambda t, a: T[t][a][1]
+    for t, k in (("customer", "c_custkey"), ("orders", "o_orderkey"), ("part", "p_partkey"),
+                 ("supplier", "s_suppkey"), ("nation", "n_nationkey"), ("region", "r_regionkey")):
+        assert len(np.unique(col(t, k))) == len(col(t, k)), t
+    pairs = col("partsupp", "ps_partkey").astype(np.int64) << 32 | col("partsupp", "ps_suppkey")
+    assert len(np.unique(pairs)) == len(pairs)
+    lpairs = col("lineitem", "l_partkey").astype(np.int64) << 32 | col("lineitem", "l_suppkey")
+    assert np.isin(lpairs, pairs).all()
+    assert np.isin(col("lineitem", "l_orderkey"), col("orders", "o_orderkey")).all()
+    assert np.isin(col("orders", "o_custkey"), col("customer", "c_custkey")).all()
+    assert (col("orders", "o_custkey") % 3 != 0).all()
+    assert (col("lineitem", "l_receiptdate") > col("lineitem", "l_shipdate")).all()
+
