@@ -30,6 +30,9 @@ WORKLOADS = {
     "c1": {"table": "fact", "kernel": "krnl_fact3", "bytes_per_row": 4 + 8 + 8 + 4,
            "desc": "filtered PK-FK hash join + group-by: dim 1.5M rows (d_cat == '2'), fact 6M rows "
                    "(1-year date window), group by d_key, SUM f_price*(1-f_disc); int32 keys, f64 sums"},
+    "c1_refill": {"table": "fact", "kernel": "krnl_fact3", "bytes_per_row": 4 + 8 + 8 + 4,
+                  "desc": "c1 with lane refill after the fact date filter (the 14% of rows that pass are "
+                          "gathered into full wavefronts before the hash probe)"},
     "c0": {"table": "cfg0", "kernel": "krnl_cfg01", "bytes_per_row": 1 + 1 + 4 + 8 + 8 + 8 + 4,
            "desc": "scan-filter-groupby (Q1 shape): date <= cutoff, group by (g1, g2), "
                    "5 f64 sums + count"},
@@ -60,7 +63,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--workload", default="c1_refill", choices=sorted(WORKLOADS) + ["c4"])
     ap.add_argument("--scale", type=float, default=10.0, help="c4: resident table scale")
     ap.add_argument("--slots", type=int, default=16, help="c4: concurrent queries")
     ap.add_argument("--arena-gb", type=float, default=48.0, help="c4: device memory for query workspaces")
@@ -80,7 +83,7 @@ def cpuBaseline(workload, g, seconds):
     from dogqc_amd import workloads
     threads = os.cpu_count() or 1
     cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
-    if workload == "c1":
+    if workload.startswith("c1"):
         run = lambda: coracle.c1(cols["dim"], cols["fact"], threads=threads)
         rows = len(cols["fact"]["f_fk"])
     elif workload.startswith("c2"):
@@ -308,7 +311,7 @@ def checkParity(workload, g, got, temps):
     if workload == "cal":
         ok = np.array_equal(np.sort(got["k_v"]), cols["k"]["k_v"])
         return {"oracle": "identity", "rows": int(len(got["k_v"])), "ok": bool(ok)}
-    if workload == "c1":
+    if workload.startswith("c1"):
         keys, sums = coracle.c1(cols["dim"], cols["fact"])
         o = np.argsort(keys)
         gk = np.asarray(got["d_key"])

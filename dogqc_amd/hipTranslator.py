@@ -29,7 +29,7 @@ from collections import OrderedDict
 
 import dogqc_amd.relationalAlgebra as ra
 from dogqc_amd.relationalAlgebra import Reduction, MaterializationType
-from dogqc_amd.types import Type
+from dogqc_amd.types import Type, COLUMN_BYTES
 from dogqc_amd.hiplang import CType, Code, langType, columnType, PACK_BITS, ident
 from dogqc_amd.hipExpr import ExprEmitter, stringConstantDecl
 from dogqc_amd.translatorBase import LiteralTranslator, UnaryTranslator, BinaryTranslator
@@ -781,18 +781,80 @@ def pipelineShape(translator):
     return loops, frac
 
 
+def firstUseFractions(scan):
+    """For each attribute a scan opens, the configured fraction of the
+    scanned rows still active at the operator that first reads it: the
+    product of the selectivities of the selections and unique-join probes
+    before that operator.  Attributes first read after a per-lane loop
+    (multimatch probe, lane refill, nested loop) are left out: they are
+    read per match, not per row."""
+    from dogqc_amd.hashJoins import EquiJoinTranslator
+    from dogqc_amd.hipDivergenceBuffer import LaneRefill
+    from dogqc_amd.relationalAlgebra import Join
+    mine = set(scan.algExpr.outRelation)
+    out = {}
+
+    def use(attrs, frac):
+        for a in attrs:
+            if a.id in mine and a.id not in out:
+                out[a.id] = frac
+
+    def exprAttrs(e):
+        found = []
+        if e is not None:
+            _collectAttrs(e, found)
+        return found
+
+    frac = 1.0
+    t = scan
+    while t.parent is not None:
+        p = t.parent
+        a = p.algExpr
+        if isinstance(p, (LaneRefill, CrossJoinTranslator)):
+            break
+        if isinstance(p, SelectionTranslator):
+            use(exprAttrs(a.condition), frac)
+            frac *= float(getattr(a, "selectivity", 1.0))
+        elif isinstance(p, EquiJoinTranslator):
+            if p.leftChild is t:          # build side: everything is stored here
+                use(list(scan.algExpr.outRelation.values()), frac)
+                break
+            use(list(a.probeKeyAttributes.values()), frac)
+            if a.multimatch and a.joinType in (Join.INNER, Join.OUTER):
+                break
+            use(exprAttrs(a.conditions), frac)
+            if a.joinType in (Join.INNER, Join.SEMI):
+                frac *= min(1.0, float(getattr(a, "selectivity", 1.0)))
+        elif isinstance(p, MapTranslator):
+            use(exprAttrs(a.expression), frac)
+        elif isinstance(p, (AggregationTranslator, MaterializeTranslator, ProjectionTranslator)):
+            use(list(scan.algExpr.outRelation.values()), frac)
+            break
+        t = p
+    return out
+
+
+def lineTouchFraction(rowFraction, elemBytes, lineBytes=128):
+    """Expected fraction of a column's cache lines read when a random
+    `rowFraction` of its rows is read."""
+    per = max(1, lineBytes // max(1, elemBytes))
+    return 1.0 - (1.0 - min(1.0, max(0.0, rowFraction))) ** per
+
+
 def rowsPerThread(translator):
     loops, _ = pipelineShape(translator)
     return 1 if loops else max(1, int(KernelCall.defaultRowsPerThread))
 
 
 class ScanTranslator(LiteralTranslator):
-    """Opens a pipeline over a base table.  Columns load lazily at first use;
-    when the configured selectivities say the filters before the first join
-    keep at least half of the rows, all columns load at once at the top
-    (lazy loading would then only add dependent memory round trips)."""
+    """Opens a pipeline over a base table.  Columns load lazily at first use,
+    except those whose cache lines would nearly all be read anyway: a column
+    first read where a fraction f of the rows is still active touches
+    1 - (1 - f)^(128 / element bytes) of its 128-byte lines, and when that is
+    at least EAGER_LINE_FRACTION the column loads at the top, with the
+    first, so its load is not one more dependent memory round trip."""
 
-    EAGER_PASS_FRACTION = 0.5
+    EAGER_LINE_FRACTION = 0.8
 
     def produce(self, ctxt):
         a = self.algExpr
@@ -824,9 +886,16 @@ class ScanTranslator(LiteralTranslator):
         self._finish(ctxt, pipe)
 
     def _finish(self, ctxt, pipe):
-        _, frac = pipelineShape(self)
-        if frac >= ScanTranslator.EAGER_PASS_FRACTION:
-            pipe.need(list(self.algExpr.outRelation.values()))
+        fracs = firstUseFractions(self)
+        eager = []
+        for id, attr in self.algExpr.outRelation.items():
+            if id not in fracs:
+                continue
+            size = 8 if attr.dataType == Type.STRING else COLUMN_BYTES.get(attr.dataType, 8)
+            if lineTouchFraction(fracs[id], size) >= ScanTranslator.EAGER_LINE_FRACTION:
+                eager.append(attr)
+        if eager:
+            pipe.need(eager)
         self.parent.consume(ctxt)
         closePipeline(ctxt, pipe)
 
@@ -1085,9 +1154,31 @@ def materializeRows(ctxt, attrs, isResult, tableName, est, opId):
             pipe.emit("stg%d[wave][sp] = %s;" % (i, c["val"]))
         pipe.code.close()
         pipe.emit("%s += wn;" % wcnt)
-    pipe.epilogue.open("if (%s > 0u)" % wcnt)
-    flush(pipe.epilogue)
-    pipe.epilogue.close()
+    # final flush, once per workgroup: one global atomic reserves the rows of
+    # all its wavefronts' stages (a per-wavefront atomic here serialises
+    # thousands of wavefronts on one counter at the end of short kernels)
+    pipe.staticLds.append("__shared__ uint32_t stg_blk[%d];" % NW)
+    pipe.staticLds.append("__shared__ unsigned long long stg_base;")
+    e = pipe.epilogue
+    e.emit("if (lane == 0u) stg_blk[wave] = %s;" % wcnt)
+    e.emit("__syncthreads();")
+    e.open("if (threadIdx.x == 0u)")
+    e.emit("uint32_t tot = 0u;")
+    e.emit("for (uint32_t w = 0; w < %du; ++w) tot += stg_blk[w];" % NW)
+    e.emit("stg_base = tot ? atomicAdd((unsigned long long*)&%s, (unsigned long long)tot) : 0ull;" % counter)
+    e.close()
+    e.emit("__syncthreads();")
+    e.open("if (%s > 0u)" % wcnt)
+    e.emit("uint64_t wb = stg_base;")
+    e.emit("for (uint32_t w = 0; w < wave; ++w) wb += stg_blk[w];")
+    e.open("for (uint32_t i = lane; i < %s; i += 64u)" % wcnt)
+    e.emit("const uint64_t wp = wb + i;")
+    e.open("if (wp < %s)" % cap)
+    for i, c in enumerate(cols):
+        e.emit("%s[wp] = stg%d[wave][i];" % (c["field"], i))
+    e.close()
+    e.close()
+    e.close()
     return tmp
 
 

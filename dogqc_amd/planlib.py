@@ -264,6 +264,7 @@ PLANS["c2_none"] = ("c2", {})
 PLANS["c2_refill"] = ("c2", {"buffers": [3]})
 PLANS["c2_pushdown"] = ("c2", {"pushdown": True})
 PLANS["c2_both"] = ("c2", {"buffers": [3], "pushdown": True})
+PLANS["c1_refill"] = ("c1", {"buffers": [4]})
 PLANS["c3"] = ("c3", {})
 PLANS["c3_refill"] = ("c3", {"buffers": [6]})
 PLANS["cal"] = ("cal", {})
@@ -305,7 +306,7 @@ def emptyAccessor(name):
 
 
 # plans the autotuner explores compile-time variants of (built ahead of time)
-TUNE_PLANS = ["c0", "c1", "c2_none", "c2_refill", "c2_pushdown", "c2_both", "c3", "c3_refill"]
+TUNE_PLANS = ["c0", "c1", "c1_refill", "c2_none", "c2_refill", "c2_pushdown", "c2_both", "c3", "c3_refill"]
 
 
 def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=None):

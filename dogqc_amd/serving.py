@@ -26,7 +26,7 @@ import numpy as np
 
 from dogqc_amd import planlib, runtime as rt, runner, workloads
 
-CONFIG_PLANS = OrderedDict([("c0", "c0"), ("c1", "c1"), ("c2", "c2_pushdown"), ("c3", "c3_refill")])
+CONFIG_PLANS = OrderedDict([("c0", "c0"), ("c1", "c1_refill"), ("c2", "c2_pushdown"), ("c3", "c3_refill")])
 SCALE_RANGE = (0.1, 10.0)
 
 

@@ -82,14 +82,16 @@ def test_c0_grid(grid):
     check("c0", workloadAcc("c0", {"cfg0": 300000}, seed=6), tuning=t, label="grid=%d" % grid)
 
 
-def test_c1_full():
-    check("c1", workloadAcc("c1"), storeKey="c1full", label="1.5M x 6M")
+@pytest.mark.parametrize("plan", ["c1", "c1_refill"])
+def test_c1_full(plan):
+    check(plan, workloadAcc("c1"), storeKey="c1full", label="1.5M x 6M")
 
 
+@pytest.mark.parametrize("plan", ["c1", "c1_refill"])
 @pytest.mark.parametrize("rows", [{"dim": 0, "fact": 1000}, {"dim": 1000, "fact": 0}, {"dim": 1, "fact": 1},
                                   {"dim": 3000, "fact": 20000}])
-def test_c1_small(rows):
-    check("c1", workloadAcc("c1", rows, seed=21), label=repr(rows))
+def test_c1_small(rows, plan):
+    check(plan, workloadAcc("c1", rows, seed=21), label=repr(rows))
 
 
 def test_c1_undersized_tables_retry():
