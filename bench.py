@@ -143,6 +143,8 @@ def main():
     ex = rt.run(mod, inputs, tuning)
     got = runner.resultColumns(mod, ex, acc)
     temps = {t["name"]: runner.tempColumns(mod, ex, acc, t["name"]) for t in mod.desc["temps"]}
+    eagerCount = ex.count
+    eagerTemps = [int(ex.res.status.temp_count[i]) for i in range(len(mod.desc["temps"]))]
     ex.release()
     parity = checkParity(args.workload, g, got, temps)
     if args.workload == "cal":
@@ -170,8 +172,16 @@ def main():
     kernel_ms = {name: statistics.mean(v) for name, v in per.items()}
 
     # the same plan captured once into a hipGraph and replayed
-    graph_ms = graphReplay(mod, inputs, tuning, ws, stream, args.steps)
-    ms_per_step = graph_ms if spec.get("graph") else eager_ms
+    graph_ms, gstatus = graphReplay(mod, inputs, tuning, ws, stream, args.steps)
+    # every replay redoes the whole plan (workspace clear included): its
+    # status must match the eager run's (no overflow, same row counts)
+    graph_ok = gstatus.flags == 0 and gstatus.result_count == eagerCount and \
+        all(gstatus.temp_count[i] == n for i, n in enumerate(eagerTemps))
+    if not graph_ok:
+        raise RuntimeError("hipGraph replay disagrees with the eager run (flags %#x, %d rows vs %d)"
+                           % (gstatus.flags, gstatus.result_count, eagerCount))
+    # both timings run the whole plan per step; the faster is reported
+    ms_per_step = min(graph_ms, eager_ms)
 
     probe_rows = rows[spec["table"]]
     dom = spec["kernel"]
@@ -202,7 +212,8 @@ def main():
         "kernels_ms": kernel_ms,
         "graph_ms_per_step": graph_ms,
         "eager_ms_per_step": eager_ms,
-        "timed": "hipGraph replay" if spec.get("graph") else "eager enqueue",
+        "timed": ("hipGraph replay (whole plan captured once; status checked against the eager run)"
+                  if graph_ms <= eager_ms else "eager enqueue"),
         "parity": parity,
         "autotuned": tuned,
     }
@@ -256,7 +267,10 @@ def benchServing(args):
 
 
 def graphReplay(mod, inputs, tuning, ws, stream, steps):
+    """Capture one enqueue of the plan into a hipGraph and time `steps`
+    replays; returns (ms per replay, the status block after the last)."""
     import ctypes as C
+    import numpy as np
     from dogqc_amd import runtime as rt
     L = rt.lib()
     gx = C.c_void_p()
@@ -271,7 +285,10 @@ def graphReplay(mod, inputs, tuning, ws, stream, steps):
     stream.sync()
     t1 = time.perf_counter()
     rt.check(L.dqc_graph_destroy(gx), "dqc_graph_destroy")
-    return (t1 - t0) * 1e3 / steps
+    # the status block heads the workspace: the last replay's flags and counts
+    raw = ws.toArray(np.uint8, C.sizeof(rt.Status))
+    status = rt.Status.from_buffer_copy(raw.tobytes())
+    return (t1 - t0) * 1e3 / steps, status
 
 
 def sameGroups(got, want, keyCols, tolCols):
