@@ -203,6 +203,35 @@ DQC_DEVICE_INLINE int64_t ht_probe(const Tag* tags, uint32_t cap, Tag tag, uint3
     return -1;
 }
 
+// A group's accumulator combined over its R copies (copy r at a[i + r*stride];
+// global group-by tables keep one copy per XCD-sized group of workgroups).
+template <typename T>
+DQC_DEVICE_INLINE T rep_sum(const T* a, int64_t i, uint32_t R, uint64_t stride) {
+    T s = a[i];
+    for (uint32_t r = 1; r < R; ++r) s += a[i + (int64_t)(r * stride)];
+    return s;
+}
+
+template <typename T>
+DQC_DEVICE_INLINE T rep_min(const T* a, int64_t i, uint32_t R, uint64_t stride) {
+    T s = a[i];
+    for (uint32_t r = 1; r < R; ++r) {
+        const T v = a[i + (int64_t)(r * stride)];
+        s = v < s ? v : s;
+    }
+    return s;
+}
+
+template <typename T>
+DQC_DEVICE_INLINE T rep_max(const T* a, int64_t i, uint32_t R, uint64_t stride) {
+    T s = a[i];
+    for (uint32_t r = 1; r < R; ++r) {
+        const T v = a[i + (int64_t)(r * stride)];
+        s = v > s ? v : s;
+    }
+    return s;
+}
+
 // The next PROBE_WINDOW slots s, s+1, ... of a unique join table, read with
 // 16-byte loads (two for 64-bit tags, one for 32-bit tags) so one memory
 // round trip resolves most probes, hits and misses alike.  Join tag buffers
@@ -505,6 +534,47 @@ DQC_DEVICE_INLINE bool string_like(str_t s, str_t p) {
     }
     while (pi < p.end && *pi == '%') ++pi;
     return pi == p.end;
+}
+
+// LIKE patterns made of literals and '%' only (no '_') are matched without
+// the general backtracking matcher: a prefix and a suffix compare bytes in
+// place, and each inner literal is found left to right by string_find, which
+// reads the string with aligned 8-byte loads and slides a register window
+// over its bytes (an aligned word holding a byte of the string lies in the
+// same page as that byte, so the loads stay in mapped memory).
+
+DQC_DEVICE_INLINE bool bytes_equal(const char* a, const char* b, uint32_t n) {
+    for (uint32_t k = 0; k < n; ++k)
+        if (a[k] != b[k]) return false;
+    return true;
+}
+
+// One past the first occurrence of lit[0..L) in [from, end), or nullptr.
+// head holds lit's first H = min(L, 8) bytes, lit[H-1] in the low byte;
+// hmask covers H bytes.
+DQC_DEVICE_INLINE const char* string_find(const char* from, const char* end, const char* lit, uint32_t L,
+                                          uint64_t head, uint64_t hmask, uint32_t H) {
+    if (end - from < (int64_t)L) return nullptr;
+    const char* last = end - L;
+    uint64_t win = 0;
+    uint32_t have = 0;
+    for (const char* p = (const char*)((uintptr_t)from & ~(uintptr_t)7); p < end; p += 8) {
+        const uint64_t w = *(const uint64_t*)p;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const char* q = p + j;
+            if (q >= from && q < end) {
+                win = (win << 8) | ((w >> (8 * j)) & 0xffu);
+                ++have;
+                if (have >= H && (win & hmask) == head) {
+                    const char* st = q - (H - 1);
+                    if (st > last) return nullptr;
+                    if (L <= H || bytes_equal(st + H, lit + H, L - H)) return st + L;
+                }
+            }
+        }
+    }
+    return nullptr;
 }
 
 // ---------------------------------------------------- multimap scan kernels

@@ -49,7 +49,12 @@ class ExprEmitter(object):
         return self._bin(e, "==")
 
     def _LikeExpr(self, e):
-        return "dqc::string_like(%s, %s)" % (self.emit(e.leftChild), self.emit(e.rightChild))
+        import dogqc_amd.scalarAlgebra as scal
+        r = e.rightChild
+        if isinstance(r, scal.ConstExpr) and r.type == Type.STRING and "_" not in r.token:
+            fn = self.program.likeMatcher(r.token)
+            return "%s(%s)" % (fn, self.emit(e.leftChild))
+        return "dqc::string_like(%s, %s)" % (self.emit(e.leftChild), self.emit(r))
 
     def _SmallerExpr(self, e):
         return self._bin(e, "<")
@@ -106,6 +111,44 @@ class ExprEmitter(object):
         for w, t in reversed(e.exprListWhenThen):
             out = "(%s ? (%s)%s : %s)" % (self.emit(w), ct, self.emit(t), out)
         return out
+
+
+def likeMatcherDecl(name, pattern, lit):
+    """Device function matching LIKE `pattern` (literals and '%' only):
+    the first segment is a prefix, the last a suffix (unless the pattern
+    starts / ends with '%'), the others are found in order in between;
+    `lit(text)` names the device constant holding a literal."""
+    segs = pattern.split("%")
+    body = ["static __device__ __forceinline__ bool %s(dqc::str_t s) {" % name]
+    if len(segs) == 1:
+        body.append("    return dqc::string_equals(s, %s);" % lit(pattern)[0])
+        body.append("}")
+        return "\n".join(body)
+    pre, mid, suf = segs[0], [m for m in segs[1:-1] if m], segs[-1]
+    n = lambda t: len(t.encode("latin-1"))
+    body.append("    const char* pos = s.start;")
+    body.append("    const char* lim = s.end;")
+    if suf:
+        body.append("    if (s.end - s.start < %d) return false;" % (n(pre) + n(suf)))
+        body.append("    lim = s.end - %d;" % n(suf))
+        body.append("    if (!dqc::bytes_equal(lim, %s, %du)) return false;" % (lit(suf)[1], n(suf)))
+    if pre:
+        body.append("    if (lim - pos < %d) return false;" % n(pre))
+        body.append("    if (!dqc::bytes_equal(pos, %s, %du)) return false;" % (lit(pre)[1], n(pre)))
+        body.append("    pos += %d;" % n(pre))
+    for m in mid:
+        b = m.encode("latin-1")
+        H = min(len(b), 8)
+        head = 0
+        for k in range(H):
+            head = (head << 8) | b[k]
+        hmask = (1 << (8 * H)) - 1
+        body.append("    pos = dqc::string_find(pos, lim, %s, %du, 0x%xull, 0x%xull, %du);"
+                    % (lit(m)[1], len(b), head, hmask, H))
+        body.append("    if (!pos) return false;")
+    body.append("    return true;")
+    body.append("}")
+    return "\n".join(body)
 
 
 def stringConstantDecl(name, token):

@@ -31,7 +31,7 @@ import dogqc_amd.relationalAlgebra as ra
 from dogqc_amd.relationalAlgebra import Reduction, MaterializationType
 from dogqc_amd.types import Type, COLUMN_BYTES
 from dogqc_amd.hiplang import CType, Code, langType, columnType, PACK_BITS, ident
-from dogqc_amd.hipExpr import ExprEmitter, stringConstantDecl
+from dogqc_amd.hipExpr import ExprEmitter, stringConstantDecl, likeMatcherDecl
 from dogqc_amd.translatorBase import LiteralTranslator, UnaryTranslator, BinaryTranslator
 from dogqc_amd.kernel import KernelCall
 from dogqc_amd.util import listWrap
@@ -318,6 +318,8 @@ class Program(object):
         self.tables = OrderedDict()      # table name -> index
         self.columns = OrderedDict()     # (table, attr, part) -> dict(index, ctype)
         self.strings = OrderedDict()
+        self.likes = OrderedDict()        # LIKE pattern -> matcher function name
+        self.likeDecls = []
         self.hts = []
         self.estimates = OrderedDict()   # opId -> C++ expression
         self.hostVars = OrderedDict()    # extra host layout statements
@@ -346,6 +348,18 @@ class Program(object):
             self.strings[token] = "dqc_lit%d" % len(self.strings)
         name = self.strings[token]
         return "dqc::str_t{%s, %s + %d}" % (name, name, len(token.encode("latin-1")))
+
+    def likeMatcher(self, pattern):
+        """Name of the generated device function matching LIKE `pattern`
+        (literals and '%' only; see hipExpr.likeMatcherDecl)."""
+        if pattern not in self.likes:
+            self.likes[pattern] = "dqc_like%d" % len(self.likes)
+
+            def lit(text):
+                full = self.stringConstant(text)
+                return full, self.strings[text]
+            self.likeDecls.append((self.likes[pattern], pattern, lit))
+        return self.likes[pattern]
 
     def newTemp(self, name, est):
         if name in self.temps:
@@ -427,9 +441,13 @@ class Program(object):
         out.emit('#include "dogqc_amd.h"')
         out.emit('#include "dqc_device.h"')
         out.emit("")
+        matchers = [likeMatcherDecl(name, pattern, lit) for name, pattern, lit in self.likeDecls]
         for token, name in self.strings.items():
             out.emit(stringConstantDecl(name, token))
         out.emit("")
+        for m in matchers:
+            out.extend(_asCode(m))
+            out.emit("")
         for pipe in self.pipes:
             out.extend(_asCode(pipe.render()))
             out.emit("")
@@ -472,6 +490,7 @@ class Program(object):
         h.emit("    int64_t rows[%d];" % max(1, nt))
         h.emit("    uint32_t ldsw[%d];" % max(1, len(self.pipes)))
         h.emit("    uint32_t ldsr[%d];" % max(1, len(self.pipes)))
+        h.emit("    uint32_t arep[%d];" % max(1, len(self.hts)))
         h.emit("};")
         h.emit("static inline uint64_t align256(uint64_t x) { return (x + 255u) & ~(uint64_t)255u; }")
         h.emit("static inline uint32_t pow2floor(uint64_t x) { uint32_t r = 1; while ((uint64_t)r * 2u <= x && r < (1u << 30)) r *= 2u; return r; }")
@@ -487,6 +506,7 @@ class Program(object):
         for opId, expr in self.estimates.items():
             h.emit("const int64_t est_%d = std::max<int64_t>(0, %s);" % (opId, expr))
             h.emit("(void)est_%d;" % opId)
+        h.emit("for (int i = 0; i < %d; ++i) L.arep[i] = 1u;" % max(1, len(self.hts)))
         for name, stmt in self.hostVars.items():
             h.emit(stmt)
         h.emit("for (int i = 0; i < %d; ++i) L.scap[i] = 0;" % max(1, len(self.hts)))
@@ -1384,10 +1404,30 @@ class AggregationTranslator(UnaryTranslator):
         fcap = pipe.param("aht%d_cap" % a.opId, "uint32_t", "L.htcap[%d]" % idx)
         self.accBuffers = []
         for acc in self.accs:
-            b = prog.buffer("agg%d_%d" % (a.opId, acc["i"]), acc["ctype"], "L.htcap[%d]%s" % (idx, extra),
+            b = prog.buffer("agg%d_%d" % (a.opId, acc["i"]), acc["ctype"],
+                            "(L.htcap[%d]%s) * (int64_t)L.arep[%d]" % (idx, extra, idx),
                             _fillPattern(acc["ctype"], _identity(acc["kind"], acc["ctype"])))
             self.accBuffers.append(b)
             acc["field"] = pipe.param("acc%d_%d" % (a.opId, acc["i"]), "%s*" % acc["ctype"], "B_" + b)
+        # copies of the accumulators, one per group of workgroups: workgroup
+        # b updates copy b % R, so with R = 8 each XCD (workgroups are dealt
+        # to the 8 XCDs round robin) updates its own lines in its own L2
+        # (tuning knob xcd_replicas; the output pass combines the copies)
+        self.repStride = "(uint64_t)(L.htcap[%d]%s)" % (idx, extra)
+        arep = pipe.param("arep%d" % a.opId, "uint32_t", "L.arep[%d]" % idx)
+        astr = pipe.param("astr%d" % a.opId, "uint64_t", self.repStride)
+        aoff = "aoff%d" % a.opId
+        pipe.prologue.emit("const uint64_t %s = (uint64_t)(blockIdx.x & (%s - 1u)) * %s;" % (aoff, arep, astr))
+        accBytes = sum(CType.size[acc["ctype"]] for acc in self.accs)
+
+        def arepVar():
+            # one copy unless the tuning asks for more: measured on c3's
+            # 106k-group table, 8 copies saved 3% in the update kernel and
+            # cost more in clearing and combining them
+            prog.hostVars["arep%d" % idx] = (
+                "{ const int32_t x = t->xcd_replicas; uint32_t R = 1u;\n"
+                "  if (x > 0 && (uint64_t)(L.htcap[%d]%s) * %du * 8u <= (512ull << 20)) R = std::min<uint32_t>(8u, pow2floor((uint64_t)x));\n"
+                "  L.arep[%d] = R; }" % (idx, extra, accBytes, idx))
         overflow = ("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
                     % (1 << idx))
 
@@ -1434,13 +1474,14 @@ class AggregationTranslator(UnaryTranslator):
                           % (gt0, gtag, gs0, ftags, fcap, gtag, gs0))
                 pipe.code.open("if (gs >= 0)")
                 for acc, v in zip(self.accs, vals):
-                    pipe.emit(_guarded(acc, "%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v)))
+                    pipe.emit(_guarded(acc, "%s;" % _atomic(acc["kind"], "%s[%s + gs]" % (acc["field"], aoff), v)))
                 if self.exact64:
                     pipe.emit("if (%s == 0) %s[%s] = 1ull;   // marks the extra slot used" % (gtag, ftags, fcap))
                 pipe.code.close()
                 pipe.emit(overflow)
                 pipe.code.close()
-            self._ldsTier(ctxt, pipe, prog, ftags, fcap, est)
+            self._ldsTier(ctxt, pipe, prog, ftags, fcap, est, aoff)
+            arepVar()
         else:
             keyFields = []
             for k, g in enumerate(groups):
@@ -1461,10 +1502,11 @@ class AggregationTranslator(UnaryTranslator):
                           "[&](uint32_t s) { return %s; });" % (ftags, fcap, writes, eqs))
                 pipe.code.open("if (gs >= 0)")
                 for acc, v in zip(self.accs, vals):
-                    pipe.emit(_guarded(acc, "%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], v)))
+                    pipe.emit(_guarded(acc, "%s;" % _atomic(acc["kind"], "%s[%s + gs]" % (acc["field"], aoff), v)))
                 pipe.code.close()
                 pipe.emit(overflow)
                 pipe.code.close()
+            arepVar()
 
     def _consumeFused(self, ctxt, pipe, prog, vals):
         """Accumulate at the slot of the probed unique join (see _fusable);
@@ -1489,7 +1531,7 @@ class AggregationTranslator(UnaryTranslator):
             pipe.emit("%s[gs] = 1;" % fhit)
             pipe.code.close()
 
-    def _ldsTier(self, ctxt, pipe, prog, ftags, fcap, est):
+    def _ldsTier(self, ctxt, pipe, prog, ftags, fcap, est, aoff):
         """Workgroup LDS table: W slots (runtime, power of two), accumulators
         replicated R times; flushed once per workgroup."""
         pid = pipe.pipeId
@@ -1542,7 +1584,7 @@ class AggregationTranslator(UnaryTranslator):
         ep.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, tg, dqc::mix64(tg), &st->flags);" % (ftags, fcap))
         ep.open("if (gs >= 0)")
         for acc in self.accs:
-            ep.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], "f%d" % acc["i"]))
+            ep.emit("%s;" % _atomic(acc["kind"], "%s[%s + gs]" % (acc["field"], aoff), "f%d" % acc["i"]))
         ep.close()
         ep.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
                 % (1 << self.ht["index"]))
@@ -1561,7 +1603,7 @@ class AggregationTranslator(UnaryTranslator):
         ep.emit("const int64_t gs = dqc::agg_slot_exact(%s, %s, tg, dqc::mix64(tg), &st->flags);" % (ftags, fcap))
         ep.open("if (gs >= 0)")
         for acc in self.accs:
-            ep.emit("%s;" % _atomic(acc["kind"], "%s[gs]" % acc["field"], "f%d" % acc["i"]))
+            ep.emit("%s;" % _atomic(acc["kind"], "%s[%s + gs]" % (acc["field"], aoff), "f%d" % acc["i"]))
         ep.close()
         ep.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
                 % (1 << self.ht["index"]))
@@ -1637,13 +1679,23 @@ class AggregationTranslator(UnaryTranslator):
         for acc in self.accs:
             if acc["kind"] == "count" and acc["in"] is None:
                 countAcc = acc
+        replicated = self.fused is None and a.doGroup
+        if replicated:
+            arep = pipe.param("arep%d" % a.opId, "uint32_t", "L.arep[%d]" % self.ht["index"])
+            astr = pipe.param("astr%d" % a.opId, "uint64_t", self.repStride)
+
+        def accValue(i):
+            if not replicated:
+                return "%s[%s]" % (accFields[i], tid)
+            fn = {"sum": "rep_sum", "count": "rep_sum", "min": "rep_min", "max": "rep_max"}[self.accs[i]["kind"]]
+            return "dqc::%s(%s, %s, %s, %s)" % (fn, accFields[i], tid, arep, astr)
+
         for id, accIdx in self.aggAcc.items():
             attr = a.aggregateAttributes[id]
             ct = langType(attr.dataType, ctxt.decimalRepr)
-            loader = "(%s)%s[%s]" % (ct, accFields[accIdx], tid)
+            loader = "(%s)%s" % (ct, accValue(accIdx))
             if id in a.avgAggregates:
-                loader = "(%s)(%s[%s] / (%s)%s[%s])" % (ct, accFields[accIdx], tid, ct,
-                                                        accFields[countAcc["i"]], tid)
+                loader = "(%s)(%s / (%s)%s)" % (ct, accValue(accIdx), ct, accValue(countAcc["i"]))
             pipe.setLazy(attr, loader, [pipe.tid], ct)
         self.parent.consume(ctxt)
         closePipeline(ctxt, pipe)

@@ -208,6 +208,20 @@ def _featurePlan(name, alg):
                                                         alg.scan("u")),
                                           alg.scan("t")))
         return p, {2: {"selectivity": 0.05}}
+    if name == "f_like_patterns":
+        # LIKE patterns of every shape the matcher generator distinguishes:
+        # inner literals (short, across words, longer than 8 bytes, several
+        # in order), prefix, suffix, prefix + suffix, exact, '%' alone, the
+        # empty pattern, and one with '_' (general matcher); counted per category
+        pats = ["%ha%", "%lpha g%", "%charlie delta%", "alpha%", "%echo", "golf", "%a%a%a%", "a%o",
+                "%", "", "%_lf%", "%ec%ho%", "%juliet juliet%", "%o%o"]
+        node = alg.scan("t")
+        aggs = []
+        for i, pat in enumerate(pats):
+            node = alg.map("m%d" % i, scal.CaseExpr([(scal.LikeExpr(_A("t_txt"), _C(pat, Type.STRING)),
+                                                      _C("1", Type.INT))], _C("0", Type.INT)), node)
+            aggs.append((Reduction.SUM, "m%d" % i, "n%d" % i))
+        return alg.aggregation(["t_cat"], aggs, node), {}
     if name == "f_outer":
         # TPC-H Q13 shape (query/plans/tpch13.py): probe rows without a
         # match count 0 through COUNT(build attribute)
@@ -254,7 +268,7 @@ def _featurePlan(name, alg):
 FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
                  "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
                  "f_groupby_int64", "f_outer", "f_outer_project", "f_join_neg", "f_anti_neg",
-                 "f_cross", "f_cross_pairs"]
+                 "f_cross", "f_cross_pairs", "f_like_patterns"]
 
 # name -> (workload or "feature", options)
 PLANS = OrderedDict()

@@ -42,7 +42,8 @@ class Tuning(C.Structure):
                 ("ht_factor", C.c_float * MAX_HTS), ("ht_capacity", C.c_int64 * MAX_HTS),
                 ("mm_entries", C.c_int64 * MAX_HTS), ("result_capacity", C.c_int64),
                 ("temp_capacity", C.c_int64 * MAX_TEMPS), ("split_capacity", C.c_int64 * MAX_HTS),
-                ("split_threshold", C.c_int32 * MAX_PIPES), ("profile", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("split_threshold", C.c_int32 * MAX_PIPES), ("profile", C.c_int32), ("xcd_replicas", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class Status(C.Structure):

@@ -343,3 +343,24 @@ def test_reference_driver_flow_with_profiler(tmp_path, monkeypatch):
     want, rplan = oracleFor("tpch3", acc)
     assertSameResult(comp.result, want, toleranceColumns(rplan), label="driver flow tpch3")
     assert "Result: %d tuples" % len(want["l_orderkey"]) in log
+
+
+@pytest.mark.parametrize("reps", [1, 2, 8])
+@pytest.mark.parametrize("name", ["f_groupby_hashed", "f_groupby_wide", "f_groupby_64", "f_outer", "f_scan_minmax"])
+def test_group_accumulator_copies(name, reps):
+    """Global group-by accumulators kept in 1, 2 or 8 copies (workgroup b
+    updates copy b % R; the output pass combines them: sums added, MIN/MAX
+    folded, AVG from the combined sum and count)."""
+    t = rt.newTuning()
+    t.xcd_replicas = reps
+    t.lds_slots[0] = -1     # no LDS tier in the first pipeline: all rows take the global path
+    t.lds_slots[1] = -1
+    check(name, planlib.featureData(20000, 700, seed=5), tuning=t, label="copies=%d" % reps)
+
+
+@pytest.mark.parametrize("reps", [1, 8])
+def test_c3_group_copies(reps):
+    t = rt.newTuning()
+    t.xcd_replicas = reps
+    check("c3_refill", workloadAcc("c3", {"sales": 200000, "item": 20000, "cust": 2000, "nation": 25}, seed=31),
+          tuning=t, label="copies=%d" % reps)
