@@ -577,6 +577,17 @@ DQC_DEVICE_INLINE const char* string_find(const char* from, const char* end, con
     return nullptr;
 }
 
+// Occupied slots of a hash table's tag array (nonzero tags), added to *out:
+// the number of keys the table holds, measured for cardinality feedback.
+template <typename Tag>
+__global__ void __launch_bounds__(256) dqc_census(const Tag* tags, uint64_t n, unsigned long long* out) {
+    uint64_t c = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
+        c += tags[i] != (Tag)0 ? 1u : 0u;
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
 // ---------------------------------------------------- multimap scan kernels
 // Bucket directory of a multimap: dir[2b] = offset of bucket b's entries,
 // dir[2b+1] = its count.  The count pass increments dir[2b+1]; these three

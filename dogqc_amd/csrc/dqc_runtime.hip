@@ -24,7 +24,7 @@
 #include "dqc_runtime_internal.h"
 
 static_assert(sizeof(dqc_plan_tuning) == 1640, "dqc_plan_tuning layout");
-static_assert(sizeof(dqc_plan_status) == 600, "dqc_plan_status layout");
+static_assert(sizeof(dqc_plan_status) == 856, "dqc_plan_status layout");
 static_assert(sizeof(dqc_plan_layout) == 1904, "dqc_plan_layout layout");
 static_assert(sizeof(dqc_plan_inputs) == 24, "dqc_plan_inputs layout");
 

@@ -592,6 +592,19 @@ class Program(object):
                 h.emit(line)
             h.close()
         h.emit("if (ev) (void)hipEventRecord((hipEvent_t)ev[%d], s);" % len(self.steps))
+        census = []
+        for ht in self.hts:
+            for prefix in ("jht", "aht"):
+                name = "%s%d_tags" % (prefix, ht["opId"])
+                if name in self.buffers:
+                    census.append((ht["index"], name, self.buffers[name]["ctype"], list(self.buffers).index(name)))
+        if census:
+            h.open("if (t->census)")
+            for i, name, ct, bi in census:
+                h.emit("hipLaunchKernelGGL(dqc::dqc_census<%s>, dim3(gridFor(0, (int64_t)L.cnt[%d], 256, 4)), "
+                       "dim3(256), 0, s, (const %s*)B_%s, (uint64_t)L.cnt[%d], "
+                       "(unsigned long long*)&st->ht_fill[%d]);" % (ct, bi, ct, name, bi, i))
+            h.close()
         h.emit("return hipGetLastError() == hipSuccess ? DQC_OK : DQC_EHIP;")
         h.close()
         return h.text()

@@ -60,6 +60,11 @@ def tempColumns(module, execution, accessor, name):
     return out
 
 
+# sizes measured by HipCompiler.execute() runs, applied to later executions
+# of the same plan module in this process (runtime.Feedback)
+FEEDBACK = rt.Feedback()
+
+
 def timeSteps(module, inputs, tuning, stream, ws):
     """Per-launch milliseconds of one enqueue (events between launches)."""
     steps = module.steps()
@@ -123,7 +128,7 @@ def executeAndPrint(compiler, printRows=10):
     if module.desc["profiles"]:
         tuning.profile = 1
     stream = rt.Stream()
-    ex = rt.run(module, inputs, tuning, stream)
+    ex = rt.run(module, inputs, tuning, stream, feedback=FEEDBACK)
     res = resultColumns(module, ex, acc)
     profiles = ex.profile()
     ws = rt.DeviceBuffer(ex.res.ws_bytes)

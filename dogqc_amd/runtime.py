@@ -7,6 +7,7 @@ fallback: if the shared library or a GPU is missing, calls raise.
 """
 import ctypes as C
 import json
+import math
 import os
 from collections import OrderedDict
 
@@ -43,13 +44,14 @@ class Tuning(C.Structure):
                 ("mm_entries", C.c_int64 * MAX_HTS), ("result_capacity", C.c_int64),
                 ("temp_capacity", C.c_int64 * MAX_TEMPS), ("split_capacity", C.c_int64 * MAX_HTS),
                 ("split_threshold", C.c_int32 * MAX_PIPES), ("profile", C.c_int32), ("xcd_replicas", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("census", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class Status(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("ht_overflow_mask", C.c_uint32), ("mm_overflow_mask", C.c_uint32),
                 ("reserved", C.c_uint32), ("result_count", C.c_uint64), ("mm_total", C.c_uint64 * MAX_HTS),
-                ("temp_count", C.c_uint64 * MAX_TEMPS), ("split_count", C.c_uint64 * MAX_HTS)]
+                ("temp_count", C.c_uint64 * MAX_TEMPS), ("split_count", C.c_uint64 * MAX_HTS),
+                ("ht_fill", C.c_uint64 * MAX_HTS)]
 
 
 class Layout(C.Structure):
@@ -399,13 +401,95 @@ def view(module, inputs, tuning, ws):
     return ex
 
 
-def run(module, inputs, tuning=None, stream=None, max_retries=8):
+class Feedback(object):
+    """Cardinality feedback: measured sizes replace the cfg estimates
+    (selectivity, htSizeFactor, numgroups; relationalAlgebra.py:342-349,
+    441-458, 564-574) on later runs of the same plan module.
+
+    A run through run(..., feedback=fb) counts the keys every unique-join and
+    group-by table really holds (tuning.census, dqc_plan_status.ht_fill);
+    with the multimap entry totals, temp-table and result row counts and
+    long-list queue lengths of the status block, learn() keeps them per
+    module together with the size of the largest input table.  apply() sets
+    explicit capacities for the next run, scaled by how much that table grew
+    or shrank: LOAD slots per key for hash tables, the counts themselves for
+    row buffers.  At the same input sizes the next run therefore needs no
+    overflow retry and allocates what the data needs."""
+
+    LOAD = 2.0
+
+    def __init__(self):
+        self.records = {}
+
+    @staticmethod
+    def key(module):
+        return os.path.basename(module.path)
+
+    def learn(self, module, inputs, execution):
+        st = execution.res.status
+        lay = execution.res.layout
+        hts = module.desc.get("hts", [])
+        self.records[self.key(module)] = {
+            "rows": int(max(inputs.rows)) if len(inputs.rows) else 0,
+            "kinds": [h["kind"] for h in hts],
+            "fill": [int(st.ht_fill[h["index"]]) for h in hts],
+            "mm": [int(st.mm_total[h["index"]]) for h in hts],
+            "split": [int(st.split_count[h["index"]]) if lay.split_capacity[h["index"]] > 0 else 0 for h in hts],
+            "temps": [int(st.temp_count[i]) for i in range(len(module.desc.get("temps", [])))],
+            "result": int(st.result_count) if module.desc.get("result") else 0,
+        }
+
+    def apply(self, module, inputs, tuning):
+        """Explicit capacities from the module's last learned run; returns
+        False when there is none."""
+        rec = self.records.get(self.key(module))
+        if rec is None:
+            return False
+        now = int(max(inputs.rows)) if len(inputs.rows) else 0
+        scale = now / float(rec["rows"]) if rec["rows"] > 0 else 1.0
+        grow = lambda n: int(math.ceil(n * scale))
+        for i, kind in enumerate(rec["kinds"]):
+            if kind == "multi":
+                # entries exactly; one bucket per entry bounds the bucket
+                # lists by the keys' own duplicates
+                if rec["mm"][i] > 0:
+                    tuning.mm_entries[i] = max(1, grow(rec["mm"][i]))
+                tuning.ht_capacity[i] = max(64, grow(rec["mm"][i]))
+            else:
+                tuning.ht_capacity[i] = max(64, int(math.ceil(rec["fill"][i] * scale * self.LOAD)))
+            if rec["split"][i] > 0:
+                tuning.split_capacity[i] = max(1, grow(rec["split"][i]))
+        for i, n in enumerate(rec["temps"]):
+            tuning.temp_capacity[i] = max(1, grow(n))
+        if module.desc.get("result"):
+            tuning.result_capacity = max(1, grow(rec["result"]))
+        return True
+
+    def save(self, path):
+        with open(path, "w") as f:
+            json.dump(self.records, f, indent=1, sort_keys=True)
+
+    def load(self, path):
+        with open(path) as f:
+            self.records.update(json.load(f))
+
+
+def run(module, inputs, tuning=None, stream=None, max_retries=8, feedback=None):
+    """Run a plan to completion (retries on overflow update `tuning` in
+    place).  With a Feedback, its capacities for this module are applied
+    first and the run's measured sizes are learned afterwards."""
     tuning = tuning or Tuning()
     stream = stream or Stream()
+    if feedback is not None:
+        feedback.apply(module, inputs, tuning)
+        tuning.census = 1
     res = QueryResult()
     check(lib().dqc_query_run(module.handle, C.byref(inputs.struct), C.byref(tuning), stream.handle,
                               max_retries, C.byref(res)), "dqc_query_run")
-    return Execution(module, inputs, tuning, res)
+    ex = Execution(module, inputs, tuning, res)
+    if feedback is not None:
+        feedback.learn(module, inputs, ex)
+    return ex
 
 
 def layout(module, inputs, tuning):

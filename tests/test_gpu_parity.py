@@ -364,3 +364,57 @@ def test_c3_group_copies(reps):
     t.xcd_replicas = reps
     check("c3_refill", workloadAcc("c3", {"sales": 200000, "item": 20000, "cust": 2000, "nation": 25}, seed=31),
           tuning=t, label="copies=%d" % reps)
+
+
+# ------------------------------------------------------------ cardinality feedback
+
+def test_census_counts_table_keys():
+    """tuning.census: ht_fill holds the keys each table really holds (the
+    dim rows passing c1's filter; the groups of a hashed group-by)."""
+    print("gpu case: census", flush=True)
+    acc = workloadAcc("c1", {"dim": 30000, "fact": 100000}, seed=41)
+    mod = module("c1")
+    t = rt.newTuning()
+    t.census = 1
+    ex = rt.run(mod, rt.BoundInputs(mod, acc), t)
+    dim = acc.column("dim", "d_cat")
+    joinIdx = [h["index"] for h in mod.desc["hts"] if h["kind"] == "unique"][0]
+    assert ex.res.status.ht_fill[joinIdx] == int(np.sum(dim == ord("2")))
+    ex.release()
+    facc = planlib.featureData(20000, 700, seed=5)
+    fmod = module("f_groupby_hashed")
+    ex = rt.run(fmod, rt.BoundInputs(fmod, facc), t)
+    aggIdx = [h["index"] for h in fmod.desc["hts"] if h["kind"] == "agg"][0]
+    assert ex.res.status.ht_fill[aggIdx] == ex.count
+    ex.release()
+
+
+@pytest.mark.parametrize("name,acc", [("c1", lambda: workloadAcc("c1", {"dim": 30000, "fact": 200000}, seed=42)),
+                                      ("f_groupby_wide", lambda: planlib.featureData(20000, 700, seed=6)),
+                                      ("tpch18", lambda: tpchAcc(0.002)), ("tpch13", lambda: tpchAcc(0.002))])
+def test_feedback_sizes_the_next_run(name, acc):
+    """A first run whose tables are far too small retries; the sizes it
+    measured let the second run (given the same bad sizes) finish without a
+    retry, with the same result."""
+    print("gpu case: feedback %s" % name, flush=True)
+    a = acc()
+    mod = module(name)
+    fb = rt.Feedback()
+
+    def tiny():
+        t = rt.newTuning()
+        for i in range(len(mod.desc["hts"])):
+            t.ht_capacity[i] = 16
+        t.result_capacity = 1
+        return t
+    ex1 = rt.run(mod, rt.BoundInputs(mod, a), tiny(), feedback=fb)
+    first = runner.resultColumns(mod, ex1, a)
+    assert ex1.res.retries > 0
+    ex1.release()
+    ex2 = rt.run(mod, rt.BoundInputs(mod, a), tiny(), feedback=fb)
+    assert ex2.res.retries == 0
+    second = runner.resultColumns(mod, ex2, a)
+    ex2.release()
+    want, plan = oracleFor(name, a)
+    assertSameResult(first, want, toleranceColumns(plan), label="%s first" % name)
+    assertSameResult(second, want, toleranceColumns(plan), label="%s second" % name)
