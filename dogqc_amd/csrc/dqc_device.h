@@ -536,6 +536,21 @@ DQC_DEVICE_INLINE bool string_like(str_t s, str_t p) {
     return pi == p.end;
 }
 
+// Batch segment of row `row`: the first member k with row < prefix[k] (the
+// members' row counts of the table, nondecreasing); members beyond the row's
+// prefix never occur because the batched run scans the largest prefix.
+DQC_DEVICE_INLINE int32_t segment_of(const int64_t* prefix, int64_t members, int64_t row) {
+    int64_t lo = 0, hi = members > 0 ? members - 1 : 0;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (row < prefix[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return (int32_t)lo;
+}
+
 // LIKE patterns made of literals and '%' only (no '_') are matched without
 // the general backtracking matcher: a prefix and a suffix compare bytes in
 // place, and each inner literal is found left to right by string_find, which

@@ -12,10 +12,11 @@ from dogqc_amd.hiplang import langType, cLiteralString
 
 class ExprEmitter(object):
 
-    def __init__(self, program, decimalRepr, access):
+    def __init__(self, program, decimalRepr, access, pipe=None):
         self.program = program
         self.decimalRepr = decimalRepr
         self.access = access  # attr -> C expression of its register
+        self.pipe = pipe
 
     def emit(self, e):
         m = getattr(self, "_" + e.__class__.__name__, None)
@@ -100,6 +101,23 @@ class ExprEmitter(object):
         if e.extract == scal.ExtractType.MONTH:
             return "((%s / 100) %% 100)" % c
         return "(%s / 10000)" % c
+
+    def _SegmentExpr(self, e):
+        """Batch segment of the scanned row: binary search of its index in
+        the members' prefix row counts of the table (batch input column
+        `__batch.<table>`, one INT64 per member, nondecreasing)."""
+        pipe = self.pipe
+        if pipe is None or getattr(pipe, "scanTable", None) != e.table:
+            raise ValueError("SegmentExpr(%r) must be mapped directly above the scan of that table" % e.table)
+        prog = self.program
+        bt = prog.table("__batch")
+        ci = prog.column("__batch", e.table, None, "int64_t")
+        f = pipe.param("c%d" % ci, "const int64_t*", "C_%d" % ci)
+        nb = pipe.param("bmembers", "int64_t", "L.rows[%d]" % bt)
+        return "dqc::segment_of(%s, %s, %s[u])" % (f, nb, pipe.tid)
+
+    def _MaxExpr(self, e):
+        return "std::max<int32_t>(%s, %s)" % (self.emit(e.leftChild), self.emit(e.rightChild))
 
     def _SubstringExpr(self, e):
         return "dqc::string_substring(%s, %s, %s)" % (self.emit(e.exprString), self.emit(e.exprFrom),

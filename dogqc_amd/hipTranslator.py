@@ -773,7 +773,7 @@ class HipCompiler(object):
 
     # helpers for translators
     def exprEmitter(self):
-        return ExprEmitter(self.program, self.decimalRepr, self.pipe.access)
+        return ExprEmitter(self.program, self.decimalRepr, self.pipe.access, self.pipe)
 
 
 # ------------------------------------------------------------------ operators
@@ -900,6 +900,7 @@ class ScanTranslator(LiteralTranslator):
         ti = prog.table(tname)
         pipe = prog.newPipeline(name, {"rows": "L.rows[%d]" % ti, "tidName": "%s%d" % (tname, a.scanTableId)},
                                 rowsPerThread(self))
+        pipe.scanTable = tname
         ctxt.pipe = pipe
         tid = pipe.tid + "[u]"
         for id, attr in a.outRelation.items():

@@ -291,6 +291,10 @@ PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_pr
 # TPC-H Q1-Q22 (dogqc_amd/tpch.py)
 for _q in range(1, 23):
     PLANS["tpch%d" % _q] = ("tpch", {"query": _q})
+# batched serving plans (dogqc_amd/batching.py): one run for several queries
+for _c in ("c0", "c1", "c2", "c3"):
+    PLANS["%s_batch" % _c] = ("batch", {"config": _c})
+
 # Q3 with a lane-refill buffer after the customer selection and a
 # lane-activity profile after the lineitem selection (profiler.py usage)
 PLANS["tpch3_profiled"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7]})
@@ -303,6 +307,12 @@ def build(name, acc):
         from dogqc_amd import tpch
         alg, p, cfg = tpch.QUERIES[opts["query"]](acc)
         return alg, p, cfg, opts
+    if kind == "batch":
+        from dogqc_amd import batching
+        alg, p, cfg, knobsOf = batching.plan(opts["config"], acc)
+        merged = dict(knobsOf)
+        merged.update(opts)
+        return alg, p, cfg, merged
     if kind == "feature":
         alg = RelationalAlgebra(acc)
         p, cfg = _featurePlan(opts.get("base", name), alg)
@@ -316,6 +326,11 @@ def emptyAccessor(name):
     if kind == "tpch":
         from dogqc_amd import tpch
         return tpch.SchemaAccessor(tpch.schema())
+    if kind == "batch":
+        from dogqc_amd import batching
+        cfgName = PLANS[name][1]["config"]
+        return batching.withBatch(workloads.emptyAccessor(cfgName),
+                                  {t: np.zeros(1, dtype=np.int64) for t in batching.SCALED[cfgName]})
     return featureEmpty() if kind == "feature" else workloads.emptyAccessor(kind)
 
 

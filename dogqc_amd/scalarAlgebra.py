@@ -256,3 +256,31 @@ class CaseExpr(ScalarExpr):
 
     def resolveType(self):
         self.type = Type.FLOAT
+
+
+# ---------------------------------------------------------------- batching
+# Extensions without a reference counterpart, used by batched plans (one run
+# of a plan for several queries that read prefixes of the same tables).
+
+class SegmentExpr(ScalarExpr):
+    """The batch segment of the current row of a scanned table: the index of
+    the first batch member whose prefix of `table` holds the row (members are
+    ordered by growing prefixes, given by the plan's `__batch` input).  Must
+    be mapped directly above the scan of `table`."""
+
+    def __init__(self, table):
+        ScalarExpr.__init__(self)
+        self.table = table
+        self.type = Type.INT
+
+    def resolve(self, algExpr):
+        return {}
+
+
+class MaxExpr(_Binary):
+    """The larger of two INT values (the segment of a joined row)."""
+
+    def resolveType(self):
+        if self.leftChild.type != Type.INT or self.rightChild.type != Type.INT:
+            raise SyntaxError("MaxExpr takes INT operands")
+        self.type = Type.INT

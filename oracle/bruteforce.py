@@ -50,7 +50,10 @@ class BruteForce(object):
                 else:
                     cols[id] = list(np.asarray(d).astype(regType(a.dataType, self.decimal)))
                 n = len(cols[id])
-            return [{id: cols[id][i] for id in cols} for i in range(n)]
+            rows = [{id: cols[id][i] for id in cols} for i in range(n)]
+            for i, r in enumerate(rows):
+                r[("__row", node.table["name"])] = i      # for SegmentExpr
+            return rows
         if isinstance(node, ra.Selection):
             return [r for r in self.rows(node.child) if self.value(node.condition, r)]
         if isinstance(node, ra.Map):

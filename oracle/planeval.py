@@ -196,6 +196,15 @@ class Evaluator(object):
                 r = a // 10000
             ct = _arith(at, "int32")
             return r.astype(ct), ct
+        if isinstance(e, scal.SegmentExpr):
+            # first batch member whose prefix of the table holds the row
+            rows = rel[("__row", e.table)]
+            prefix = np.asarray(self.acc.column("__batch", e.table), dtype=np.int64)
+            return np.searchsorted(prefix, rows, side="right").astype(np.int32), "int32_t"
+        if isinstance(e, scal.MaxExpr):
+            a, _ = self.expr(e.leftChild, rel, n)
+            b, _ = self.expr(e.rightChild, rel, n)
+            return np.maximum(np.asarray(a, dtype=np.int32), np.asarray(b, dtype=np.int32)), "int32_t"
         if isinstance(e, scal.SubstringExpr):
             s, _ = self.expr(e.exprString, rel, n)
             f, _ = self.expr(e.exprFrom, rel, n)
@@ -269,6 +278,7 @@ class Evaluator(object):
             n = len(rel[id])
         if n is None:
             n = node.table["size"]
+        rel[("__row", tname)] = np.arange(n, dtype=np.int64)   # for SegmentExpr
         return rel, n
 
     def _Selection(self, node):

@@ -418,3 +418,32 @@ def test_feedback_sizes_the_next_run(name, acc):
     want, plan = oracleFor(name, a)
     assertSameResult(first, want, toleranceColumns(plan), label="%s first" % name)
     assertSameResult(second, want, toleranceColumns(plan), label="%s second" % name)
+
+
+# ------------------------------------------------------------ batched plans
+
+@pytest.mark.parametrize("config", ["c0", "c1", "c2", "c3"])
+def test_batched_plan_on_gpu(config):
+    """One GPU run of a batched plan over five members' prefixes, combined
+    per member on the host, equals each member's own query on the oracle."""
+    from dogqc_amd import batching
+    from tests.test_batching import SIZES, SCALES, prefixes, oracleMember
+    print("gpu case: batched %s" % config, flush=True)
+    sizes = {t: n * 10 for t, n in SIZES[config].items()}
+    if config == "c3":
+        sizes["nation"] = 25
+    acc = MemoryAccessor(workloads.generate(config, sizes, seed=9))
+    pref = {t: [max(1, int(round(sizes[t] * s))) for s in SCALES] for t in batching.SCALED[config]}
+    bacc = batching.batchAccessor(acc, pref)
+    mod = module("%s_batch" % config)
+    ex = rt.run(mod, rt.BoundInputs(mod, bacc), rt.newTuning())
+    got = runner.resultColumns(mod, ex, bacc)
+    temps = {t["name"]: runner.tempColumns(mod, ex, bacc, t["name"]) for t in mod.desc["temps"]}
+    ex.release()
+    for k in range(len(SCALES)):
+        want, wantTemps, plan = oracleMember(config, acc, pref, k)
+        res, tmp = batching.combine(config, got, temps, k)
+        assertSameResult(res, want, toleranceColumns(plan), label="%s member %d" % (config, k))
+        for name, cols in tmp.items():
+            w = {c: wantTemps[name][c] for c in cols}
+            assertSameResult(cols, w, {"revenue"}, label="%s member %d %s" % (config, k, name))

@@ -55,6 +55,12 @@ def test_planeval_matches_bruteforce(name):
         acc = planlib.featureData(160, 40, seed=3)
     elif kind == "tpch":
         acc = tpchAccessor()
+    elif kind == "batch":
+        from dogqc_amd import batching
+        config = planlib.PLANS[name][1]["config"]
+        full = MemoryAccessor(workloads.generate(config, SMALL[config], seed=5))
+        pref = {t: [max(1, SMALL[config][t] // 3), SMALL[config][t]] for t in batching.SCALED[config]}
+        acc = batching.batchAccessor(full, pref)
     else:
         acc = MemoryAccessor(workloads.generate(kind, SMALL[kind], seed=5))
     plan = resolved(name, acc)
