@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--slots", type=int, default=16, help="c4: concurrent queries")
     ap.add_argument("--arena-gb", type=float, default=48.0, help="c4: device memory for query workspaces")
     ap.add_argument("--max-heavy", type=int, default=1, help="c4: heavy queries running at once (0 = no limit)")
+    ap.add_argument("--max-batch", type=int, default=16,
+                    help="c4: waiting queries of one config run as one batched plan, up to this many (<= 1: off)")
     ap.add_argument("--heavy-mb", type=float, default=64.0, help="c4: workspace size of a heavy query")
     ap.add_argument("--rows-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -233,7 +235,7 @@ def benchServing(args):
     nq = max(8, args.steps) if args.steps != 50 else 200
     log = lambda m: print("[c4] " + m, file=sys.stderr, flush=True)
     r = serving.benchmark(scale=args.scale, nqueries=nq, slots=args.slots, arena_gb=args.arena_gb, log=log,
-                          max_heavy=args.max_heavy, heavy_mb=args.heavy_mb)
+                          max_heavy=args.max_heavy, heavy_mb=args.heavy_mb, max_batch=args.max_batch)
     out = {
         "metric": METRIC,
         "value": r["batched"]["qps"],
@@ -249,7 +251,7 @@ def benchServing(args):
         "data": "synthetic (generated on the device, dogqc_amd/datagen.py)",
         "config": {"workload": SERVING["desc"], "resident_gb": r["resident_gb"], "scale": r["scale"],
                    "queries": nq, "slots": r["slots"], "arena_gb": r["arena_gb"], "max_heavy": r["max_heavy"],
-                   "heavy_mb": r["heavy_mb"], "plans": r["plans"],
+                   "heavy_mb": r["heavy_mb"], "max_batch": r["max_batch"], "plans": r["plans"],
                    "parallelism": "single GPU"},
         "roofline": None,
         "serial_qps": r["serial"]["qps"],
@@ -259,7 +261,11 @@ def benchServing(args):
         "latency_at_offered_qps": {"serial": {k: r["serial_at_rate"][k] for k in ("p50_ms", "p99_ms", "mean_ms")},
                                    "batched": {k: r["batched_at_rate"][k] for k in ("p50_ms", "p99_ms", "mean_ms")}},
         "capacity_runs": {"serial": r["serial"], "batched": r["batched"]},
+        "batching": {"capacity_run": {k: r["batched"][k] for k in ("in_batches", "mean_batch")},
+                     "at_offered_qps": {k: r["batched_at_rate"][k] for k in ("in_batches", "mean_batch")}},
         "by_config": {"serial": r["serial_by_config"], "batched": r["batched_by_config"]},
+        "latency_at_offered_qps_by_config": {"serial": r["serial_at_rate_by_config"],
+                                             "batched": r["batched_at_rate_by_config"]},
         "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
         "parity": r["verify"],
     }

@@ -24,6 +24,12 @@ from dogqc_amd.types import Type
 
 A, C = scal.AttrExpr, scal.ConstExpr
 
+# group-count hints sized for batches of up to this many members: the
+# segment multiplies a query's groups, and a hint that is too small costs
+# retries while one too large only costs workspace (small group-bys stay on
+# the LDS tier; see AggregationTranslator)
+MEMBERS = 16
+
 # tables whose prefix grows with a query's scale, per config (c3's 25 nations
 # are always read whole)
 SCALED = {"c0": ["cfg0"], "c1": ["dim", "fact"], "c2": ["build", "probe"], "c3": ["item", "cust", "sales"]}
@@ -49,7 +55,7 @@ def plan(config, acc):
                             [(Reduction.SUM, "qty", "sum_qty"), (Reduction.SUM, "price", "sum_base_price"),
                              (Reduction.SUM, "disc_price", "sum_disc_price"), (Reduction.SUM, "charge", "sum_charge"),
                              (Reduction.SUM, "disc", "sum_disc"), (Reduction.COUNT, "", "count_order")], dp)
-        return alg, p, {}, {}
+        return alg, p, {p.opId: {"numgroups": 6 * MEMBERS}}, {}
     if config == "c1":
         dim = alg.selection(scal.EqualsExpr(A("d_cat"), C("2", Type.CHAR)), _seg(alg, "seg_d", "dim", alg.scan("dim")))
         fact = alg.selection(scal.AndExpr(scal.LargerEqualExpr(A("f_date"), C("19950101", Type.DATE)),
@@ -69,6 +75,7 @@ def plan(config, acc):
         prod = alg.map("prod", scal.MulExpr(A("b_pay"), A("p_pay")),
                        alg.map("seg", scal.MaxExpr(A("seg_b"), A("seg_p")), j))
         p = alg.aggregation(["seg"], [(Reduction.COUNT, "", "cnt"), (Reduction.SUM, "prod", "sum_prod")], prod)
+        cfg[p.opId] = {"numgroups": MEMBERS}
         return alg, p, cfg, {"pushdown": True}
     if config == "c3":
         like = scal.LikeExpr(A("i_text"), C("%" + workloads.C3_WORD + "%", Type.STRING))
@@ -81,11 +88,17 @@ def plan(config, acc):
                       alg.map("seg", scal.MaxExpr(A("seg_is"), A("seg_c")), j2))
         star = alg.createtemp("c3_star", alg.projection(["seg", "i_key", "n_key", "c_year", "s_qty", "s_price"], j3))
         aggs = [(Reduction.SUM, "s_qty", "sum_qty"), (Reduction.SUM, "s_price", "revenue"), (Reduction.COUNT, "", "cnt")]
-        ny = alg.createtemp("c3_by_nation_year", alg.aggregation(["seg", "n_key", "c_year"], aggs, alg.scan("c3_star")))
+        # (segment, year) packed into one INT so that the group key is 64
+        # bits (exact tags, LDS tier): year * 64 + segment
+        yseg = alg.map("yseg", scal.AddExpr(scal.MulExpr(A("c_year"), C("64", Type.INT)), A("seg")),
+                       alg.scan("c3_star"))
+        ny = alg.createtemp("c3_by_nation_year", alg.aggregation(["yseg", "n_key"], aggs, yseg))
         byItem = alg.aggregation(["seg", "i_key"], aggs, alg.scan("c3_star"))
+        # (nation, year) groups stay on the LDS tier (2000 -> 4096 slots);
+        # (segment, item) groups are bounded by the star rows (no hint: sized
+        # from the temp table's row estimate, so large batches do not retry)
         cfg = {j1.opId: {"multimatch": False, "selectivity": 0.06}, j2.opId: {"multimatch": False},
-               j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 175},
-               byItem.opId: {"numgroups": 120000}}
+               j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 2000}}
         return alg, [star, ny, byItem], cfg, {"buffers": [j1.opId]}
     raise KeyError(config)
 
@@ -189,6 +202,9 @@ def combine(config, result, temps, member):
         return f, {}
     if config == "c3":
         sums = ["sum_qty", "revenue", "cnt"]
+        ny = OrderedDict(temps["c3_by_nation_year"])
+        yseg = np.asarray(ny.pop("yseg")).astype(np.int64)
+        ny["seg"], ny["c_year"] = (yseg % 64).astype(np.int32), (yseg // 64).astype(np.int32)
         return (_fold(result, member, ["i_key"], sums=sums),
-                {"c3_by_nation_year": _fold(temps["c3_by_nation_year"], member, ["n_key", "c_year"], sums=sums)})
+                {"c3_by_nation_year": _fold(ny, member, ["n_key", "c_year"], sums=sums)})
     raise KeyError(config)

@@ -551,6 +551,16 @@ DQC_DEVICE_INLINE int32_t segment_of(const int64_t* prefix, int64_t members, int
     return (int32_t)lo;
 }
 
+// The same for a row of the workgroup tile [base, base + tile): when the
+// tile's first and last rows share a segment (almost every tile) the
+// workgroup-uniform answer avoids a search per lane.
+DQC_DEVICE_INLINE int32_t segment_of(const int64_t* prefix, int64_t members, int64_t row, int64_t base,
+                                     int64_t tile) {
+    const int32_t s0 = segment_of(prefix, members, base);
+    const int32_t s1 = segment_of(prefix, members, base + tile - 1);
+    return s0 == s1 ? s0 : segment_of(prefix, members, row);
+}
+
 // LIKE patterns made of literals and '%' only (no '_') are matched without
 // the general backtracking matcher: a prefix and a suffix compare bytes in
 // place, and each inner literal is found left to right by string_find, which

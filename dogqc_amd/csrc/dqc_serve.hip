@@ -35,8 +35,8 @@
 #include "dqc_runtime_internal.h"
 
 static_assert(sizeof(dqc_serve_config) == 48, "dqc_serve_config layout");
-static_assert(sizeof(dqc_serve_request) == 56, "dqc_serve_request layout");
-static_assert(sizeof(dqc_serve_record) == 56 + sizeof(dqc_plan_layout) + sizeof(dqc_plan_status),
+static_assert(sizeof(dqc_serve_request) == 96, "dqc_serve_request layout");
+static_assert(sizeof(dqc_serve_record) == 64 + sizeof(dqc_plan_layout) + sizeof(dqc_plan_status),
               "dqc_serve_record layout");
 
 namespace {
@@ -107,6 +107,21 @@ struct Request {
     bool keep = false;
     bool heavy = false;          // counted against max_heavy while running
 
+    // a query that may run in a batch (see dqc_serve_request)
+    Module* batchModule = nullptr;
+    std::vector<const void*> batchCols;
+    std::vector<int32_t> batchColPrefix;
+    std::vector<int32_t> batchTableMap;
+    int64_t batchOf = -1;        // the batch it ran in
+
+    // a batch: a request of the server's own for several queries
+    bool internal = false;
+    std::vector<int64_t> members;       // query ids, by growing prefixes
+    std::vector<int32_t> prefixCols;    // batched-module columns filled with the members' prefixes
+    std::vector<int64_t> prefix;        // [prefix column][member] row counts (host copy)
+    uint64_t prefixOff = 0;             // prefix block, after the workspace in the allocation
+    int keepRefs = 0;                   // members that keep the result and have not released it
+
     void bindInputs() {
         inputs.table_rows = rows.data();
         inputs.cols = cols.data();
@@ -165,7 +180,82 @@ struct Server {
         r.rec.state = state;
         r.rec.error = error;
         r.rec.done_ns = now();
-        --unfinished;
+        if (!r.internal) {
+            --unfinished;
+            return;
+        }
+        // a batch ends its members with it
+        for (size_t k = 0; k < r.members.size(); ++k) {
+            Request& m = *reqs[r.members[k]];
+            m.rec.batch_size = (int32_t)r.members.size();
+            m.rec.batch_index = (int32_t)k;
+            m.rec.retries = r.rec.retries;
+            m.rec.layout = r.rec.layout;
+            m.rec.status = r.rec.status;
+            m.rec.ws_bytes = r.rec.ws_bytes;
+            m.rec.ws = (state == DQC_SERVE_DONE && m.keep) ? r.rec.ws : nullptr;
+            finish(m, state, error);
+        }
+    }
+
+    static constexpr uint64_t align256(uint64_t x) { return (x + 255u) & ~(uint64_t)255u; }
+
+    // A batch of the waiting queries `group` (same module and batched
+    // module), or -1 when their prefixes are not nested in every table or
+    // the batch would not fit the arena.
+    int64_t makeBatch(std::vector<int64_t> group) {
+        std::stable_sort(group.begin(), group.end(), [this](int64_t a, int64_t b) {
+            return drivingRows(*reqs[a]) < drivingRows(*reqs[b]);
+        });
+        const Request& m0 = *reqs[group[0]];
+        const size_t N = group.size(), nt = m0.batchTableMap.size(), nc = m0.batchCols.size();
+        std::unique_ptr<Request> b(new Request());
+        b->internal = true;
+        b->module = m0.batchModule;
+        b->members = group;
+        b->rows.assign(nt, 0);
+        for (size_t j = 0; j < nt; ++j) {
+            const int32_t src = m0.batchTableMap[j];
+            if (src < 0) {
+                b->rows[j] = (int64_t)N;
+                continue;
+            }
+            int64_t prev = 0;
+            for (int64_t id : group) {
+                const Request& m = *reqs[id];
+                if (src >= (int32_t)m.rows.size()) return -1;
+                const int64_t v = m.rows[src];
+                if (v < prev) return -1;   // prefixes not nested: no batch
+                prev = v;
+            }
+            b->rows[j] = prev;
+        }
+        b->cols = m0.batchCols;
+        for (size_t c = 0; c < nc; ++c) {
+            const int32_t j = m0.batchColPrefix[c];
+            if (j < 0) continue;
+            if (j >= (int32_t)nt || m0.batchTableMap[j] < 0) return -1;
+            b->prefixCols.push_back((int32_t)c);
+            for (int64_t id : group) b->prefix.push_back(reqs[id]->rows[m0.batchTableMap[j]]);
+        }
+        b->inputs.ntables = (int32_t)nt;
+        b->inputs.ncols = (int32_t)nc;
+        b->bindInputs();
+        int64_t arrival = m0.rec.arrival_ns;
+        for (int64_t id : group) {
+            arrival = std::min(arrival, reqs[id]->rec.arrival_ns);
+            b->keep = b->keep || reqs[id]->keep;
+        }
+        b->rec.arrival_ns = arrival;
+        b->rec.state = DQC_SERVE_PENDING;
+        b->rec.slot = -1;
+        dqc_plan_layout L;
+        if (b->module->layout(&b->inputs, &b->tuning, &L)) return -1;
+        if (align256(L.ws_bytes) + b->prefix.size() * sizeof(int64_t) > cfg.arena_bytes) return -1;
+        const int64_t id = (int64_t)reqs.size();
+        b->id = id;
+        reqs.push_back(std::move(b));
+        return id;
     }
 
     void freeWs(Request& r) {
@@ -252,7 +342,9 @@ struct Server {
         if (heavy && runningHeavy() >= cfg.max_heavy) return HELD_HEAVY;
         if (!heavy && heavyWaiting && freeSlots() <= 1) return HELD_LIGHT;
         uint64_t off = 0;
-        if (!arena->alloc(L.ws_bytes, &off)) return NO_MEMORY;
+        const uint64_t prefixBytes = r.prefix.size() * sizeof(int64_t);
+        const uint64_t allocBytes = prefixBytes ? align256(L.ws_bytes) + prefixBytes : L.ws_bytes;
+        if (!arena->alloc(allocBytes, &off)) return NO_MEMORY;
         r.arenaOff = off;
         r.hasWs = true;
         r.heavy = heavy;
@@ -263,8 +355,19 @@ struct Server {
         r.rec.slot = si;
         r.rec.state = DQC_SERVE_RUNNING;
         Slot& s = slots[si];
-        rc = r.module->enqueue(&r.inputs, &r.tuning, r.rec.ws, L.ws_bytes, (void*)s.stream, nullptr);
         hipError_t e = hipSuccess;
+        if (prefixBytes) {
+            // the members' prefix row counts after the workspace; the copy
+            // precedes the plan on the slot's stream
+            r.prefixOff = align256(L.ws_bytes);
+            int64_t* dev = (int64_t*)((char*)r.rec.ws + r.prefixOff);
+            const size_t N = r.members.size();
+            for (size_t i = 0; i < r.prefixCols.size(); ++i) r.cols[r.prefixCols[i]] = dev + i * N;
+            r.bindInputs();
+            e = hipMemcpyAsync(dev, r.prefix.data(), prefixBytes, hipMemcpyHostToDevice, s.stream);
+        }
+        rc = e == hipSuccess ? r.module->enqueue(&r.inputs, &r.tuning, r.rec.ws, L.ws_bytes, (void*)s.stream, nullptr)
+                             : DQC_EHIP;
         if (rc == DQC_OK) e = hipMemcpyAsync(&pinned[si], r.rec.ws, sizeof(dqc_plan_status), hipMemcpyDeviceToHost,
                                             s.stream);
         if (rc == DQC_OK && e == hipSuccess) e = hipEventRecord(s.done, s.stream);
@@ -315,6 +418,10 @@ struct Server {
                 }
                 continue;
             }
+            if (r.internal) {
+                r.keepRefs = 0;
+                for (int64_t id : r.members) r.keepRefs += reqs[id]->keep ? 1 : 0;
+            }
             if (!r.keep) freeWs(r);
             if (cfg.size_feedback && r.rec.retries > 0) learn(r);
             finish(r, DQC_SERVE_DONE, 0);
@@ -328,6 +435,36 @@ struct Server {
             if (si < 0) break;
             Request& r = *reqs[*it];
             if (r.rec.arrival_ns > ta) break;   // pending is kept in arrival order
+            if (cfg.max_batch > 1 && r.batchModule && !r.internal) {
+                // waiting queries of the same plan join this one in a batch
+                std::vector<int64_t> group{*it};
+                const int maxb = std::min(cfg.max_batch, DQC_SERVE_MAX_BATCH);
+                for (auto jt = std::next(it); jt != pending.end() && (int)group.size() < maxb; ++jt) {
+                    const Request& o = *reqs[*jt];
+                    if (o.rec.arrival_ns > ta) break;
+                    if (!o.internal && o.module == r.module && o.batchModule == r.batchModule) group.push_back(*jt);
+                }
+                const int64_t bid = group.size() >= 2 ? makeBatch(group) : -1;
+                if (bid >= 0) {
+                    const Admit a = tryAdmit(bid, si, ta, heavyWaiting);
+                    if (a == ADMITTED) {
+                        progress = true;
+                        for (int64_t id : group) {
+                            Request& m = *reqs[id];
+                            m.batchOf = bid;
+                            m.rec.state = reqs[bid]->rec.state;
+                            m.rec.admit_ns = ta;
+                            m.rec.slot = si;
+                        }
+                        for (auto kt = pending.begin(); kt != pending.end();)
+                            kt = std::find(group.begin(), group.end(), *kt) != group.end() ? pending.erase(kt)
+                                                                                          : std::next(kt);
+                        it = pending.begin();
+                        continue;
+                    }
+                    reqs.pop_back();   // not admitted now: the queries wait as they are
+                }
+            }
             const Admit a = tryAdmit(*it, si, ta, heavyWaiting);
             if (a == ADMITTED) {
                 progress = true;
@@ -429,6 +566,15 @@ int dqc_serve_submit(void* server, const dqc_serve_request* req, int64_t* id) {
     r->bindInputs();
     if (req->tuning) r->tuning = *req->tuning;
     r->keep = req->keep_result != 0;
+    if (req->batch_module) {
+        if (req->batch_ncols <= 0 || req->batch_ntables <= 0 || !req->batch_cols || !req->batch_col_prefix ||
+            !req->batch_table_map)
+            return fail(DQC_EINVAL, "dqc_serve_submit: incomplete batch binding");
+        r->batchModule = (Module*)req->batch_module;
+        r->batchCols.assign(req->batch_cols, req->batch_cols + req->batch_ncols);
+        r->batchColPrefix.assign(req->batch_col_prefix, req->batch_col_prefix + req->batch_ncols);
+        r->batchTableMap.assign(req->batch_table_map, req->batch_table_map + req->batch_ntables);
+    }
     r->rec.arrival_ns = req->arrival_ns;
     r->rec.state = DQC_SERVE_PENDING;
     r->rec.slot = -1;
@@ -483,6 +629,20 @@ int dqc_serve_release(void* server, int64_t id) {
     if (id < 0 || id >= (int64_t)s->reqs.size()) return fail(DQC_EINVAL, "no query %lld", (long long)id);
     Request& r = *s->reqs[id];
     if (r.rec.state != DQC_SERVE_DONE) return fail(DQC_EINVAL, "query %lld is not done", (long long)id);
+    if (r.batchOf >= 0) {
+        // the batch's workspace goes when its last keeping member lets go
+        Request& b = *s->reqs[r.batchOf];
+        if (r.rec.ws) {
+            r.rec.ws = nullptr;
+            if (--b.keepRefs <= 0 && b.hasWs) {
+                if (s->started.load() && !s->stop.load())
+                    s->toRelease.push_back(b.id);
+                else
+                    s->freeWs(b);
+            }
+        }
+        return DQC_OK;
+    }
     if (!r.hasWs) return DQC_OK;
     if (s->started.load() && !s->stop.load())
         s->toRelease.push_back(id);   // the scheduler thread owns the arena

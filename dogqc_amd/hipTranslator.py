@@ -496,6 +496,16 @@ class Program(object):
         h.emit("static inline uint32_t pow2floor(uint64_t x) { uint32_t r = 1; while ((uint64_t)r * 2u <= x && r < (1u << 30)) r *= 2u; return r; }")
         h.emit("static inline uint32_t pow2ceil(uint64_t x) { uint32_t r = 1; while ((uint64_t)r < x && r < (1u << 30)) r *= 2u; return r; }")
         h.emit("static inline float fac(float f) { return f > 0.0f ? f : 1.0f; }")
+        h.emit("static int g_lds = 0;")
+        h.open("static uint64_t maxLds()")
+        h.open("if (g_lds == 0)")
+        h.emit("int dev = 0; (void)hipGetDevice(&dev);")
+        h.emit("int b = 0;")
+        h.emit("if (hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || b <= 0) b = 65536;")
+        h.emit("g_lds = b;")
+        h.close()
+        h.emit("return (uint64_t)g_lds;")
+        h.close()
         h.open("static int computeLayout(const dqc_plan_inputs* in, const dqc_plan_tuning* t, Layout& L)")
         h.emit("if (!in || !t) return DQC_EINVAL;")
         h.emit("if (in->ntables != %d || in->ncols != %d) return DQC_EINVAL;" % (nt, nc))
@@ -672,10 +682,19 @@ def pipelineLaunch(prog, pipe, phase=None, rowsExpr=None, perCu=8):
     for f, (ct, expr) in pipe.params.items():
         lines.append("kp.%s = (%s)(%s);" % (f, ct, expr))
     rows = rowsExpr or pipe.params["n"][1]
-    lines.append("const unsigned g = gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, %d);"
-                 % (pipe.pipeId, pipe.pipeId, pipe.defaultGrid, rows, pipe.bs * pipe.U, perCu))
+    per = str(perCu)
+    if "lw" in pipe.params:
+        # an LDS group table is flushed once per workgroup: with many slots,
+        # fewer workgroups (8 per CU at <= 512 slots, 1 per CU at >= 4096)
+        per = "std::max<int>(1, (int)(%d * 512u / std::max<uint32_t>(512u, L.ldsw[%d])))" % (perCu, pipe.pipeId)
+    lines.append("const unsigned g = gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, %s);"
+                 % (pipe.pipeId, pipe.pipeId, pipe.defaultGrid, rows, pipe.bs * pipe.U, per))
     lds = "0" if pipe.dynLdsExpr is None else "(size_t)(%s)" % pipe.dynLdsExpr
     kname = pipe.name if phase is None else "%s<%d>" % (pipe.name, phase)
+    if pipe.dynLdsExpr is not None:
+        # dynamic LDS beyond 64 KB (gfx950 has 160 KB per workgroup) is opted into per kernel
+        lines.append("if (%s > 65536u) (void)hipFuncSetAttribute((const void*)&%s, "
+                     "hipFuncAttributeMaxDynamicSharedMemorySize, (int)(%s));" % (lds, kname, lds))
     lines.append("hipLaunchKernelGGL((%s), dim3(g), dim3(%d), %s, s, kp);" % (kname, pipe.bs, lds))
     return lines
 
@@ -812,6 +831,31 @@ def pipelineShape(translator):
             break
         t = p
     return loops, frac
+
+
+_SHARED_DECL = None
+
+
+def staticLdsBytes(pipe):
+    """Bytes of a pipeline's static __shared__ arrays (its declarations)."""
+    import re
+    global _SHARED_DECL
+    if _SHARED_DECL is None:
+        _SHARED_DECL = re.compile(r"__shared__\s+(.+?)\s+\w+((?:\[\d+\])*)\s*;")
+    sizes = {"uint8_t": 1, "int8_t": 1, "char": 1, "int16_t": 2, "uint16_t": 2, "int32_t": 4, "uint32_t": 4,
+             "float": 4, "int64_t": 8, "uint64_t": 8, "unsigned long long": 8, "double": 8, "dqc::str_t": 16,
+             "dqc::str_offs": 16}
+    total = 0
+    for d in pipe.staticLds:
+        m = _SHARED_DECL.search(d)
+        if not m:
+            total += 1024
+            continue
+        n = 1
+        for dim in re.findall(r"\[(\d+)\]", m.group(2)):
+            n *= int(dim)
+        total += n * sizes.get(m.group(1).strip(), 8)
+    return total
 
 
 def firstUseFractions(scan):
@@ -1559,10 +1603,12 @@ class AggregationTranslator(UnaryTranslator):
             "    int64_t r = t->lds_replicas[%d];\n"
             "    if (r > 0) R = std::min<uint32_t>(64u, pow2floor((uint64_t)r));\n"
             "    else { R = 64u; while (R > 1u && (uint64_t)W * (8u + (uint64_t)R * %du) > %du) R /= 2u; }\n"
-            "    if ((uint64_t)W * (8u + (uint64_t)R * %du) > 65536u) W = 0;\n"
+            "    while (W > 16u && (uint64_t)W * (8u + (uint64_t)R * %du) + %du > maxLds()) W /= 2u;\n"
+            "    if ((uint64_t)W * (8u + (uint64_t)R * %du) + %du > maxLds()) W = 0;\n"
             "  }\n"
             "  L.ldsw[%d] = W; L.ldsr[%d] = W ? R : 1u; }"
-            % (pid, est, pid, accBytes, LDS_AGG_BUDGET, accBytes, pid, pid))
+            % (pid, est, pid, accBytes, LDS_AGG_BUDGET, accBytes, staticLdsBytes(pipe) + 1024,
+               accBytes, staticLdsBytes(pipe) + 1024, pid, pid))
         pipe.param("lw", "uint32_t", "L.ldsw[%d]" % pid)
         pipe.param("lr", "uint32_t", "L.ldsr[%d]" % pid)
         # dynamic LDS: tags, then one region per accumulator (8-byte aligned)

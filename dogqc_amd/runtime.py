@@ -71,18 +71,22 @@ class QueryResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_retries", C.c_int32), ("arena_bytes", C.c_uint64),
                 ("bypass_ms", C.c_int32), ("size_feedback", C.c_int32), ("max_heavy", C.c_int32),
-                ("heavy_bytes", C.c_uint64), ("reserved", C.c_int32 * 2)]
+                ("heavy_bytes", C.c_uint64), ("max_batch", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ServeRequest(C.Structure):
     _fields_ = [("module", C.c_void_p), ("inputs", Inputs), ("tuning", C.POINTER(Tuning)),
-                ("arrival_ns", C.c_int64), ("keep_result", C.c_int32), ("reserved", C.c_int32)]
+                ("arrival_ns", C.c_int64), ("keep_result", C.c_int32), ("batch_ntables", C.c_int32),
+                ("batch_module", C.c_void_p), ("batch_cols", C.POINTER(C.c_void_p)),
+                ("batch_col_prefix", C.POINTER(C.c_int32)), ("batch_table_map", C.POINTER(C.c_int32)),
+                ("batch_ncols", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ServeRecord(C.Structure):
     _fields_ = [("arrival_ns", C.c_int64), ("admit_ns", C.c_int64), ("done_ns", C.c_int64),
                 ("state", C.c_int32), ("retries", C.c_int32), ("error", C.c_int32), ("slot", C.c_int32),
-                ("ws", C.c_void_p), ("ws_bytes", C.c_uint64), ("layout", Layout), ("status", Status)]
+                ("ws", C.c_void_p), ("ws_bytes", C.c_uint64), ("batch_size", C.c_int32),
+                ("batch_index", C.c_int32), ("layout", Layout), ("status", Status)]
 
 
 class DqcError(RuntimeError):

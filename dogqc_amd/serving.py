@@ -34,15 +34,19 @@ class Server(object):
     """ctypes wrapper of one dqc_serve server."""
 
     def __init__(self, slots, arena_bytes, max_retries=8, bypass_ms=50, size_feedback=True, max_heavy=0,
-                 heavy_bytes=0):
+                 heavy_bytes=0, max_batch=0):
         cfg = rt.ServeConfig()
         cfg.slots, cfg.max_retries, cfg.arena_bytes, cfg.bypass_ms = slots, max_retries, int(arena_bytes), bypass_ms
         cfg.size_feedback = 1 if size_feedback else 0
         cfg.max_heavy, cfg.heavy_bytes = int(max_heavy), int(heavy_bytes)
+        cfg.max_batch = int(max_batch)
         self.handle = C.c_void_p()
+        self._alive = []      # host arrays the server reads after submit
         rt.check(rt.lib().dqc_serve_create(C.byref(cfg), C.byref(self.handle)), "dqc_serve_create")
 
-    def submit(self, module, rows, cols, tuning=None, arrival_ns=0, keep=False):
+    def submit(self, module, rows, cols, tuning=None, arrival_ns=0, keep=False, batch=None):
+        """Queue a query; `batch` = (batched module, its BatchBinding) lets
+        the server run it together with other queries of the same plan."""
         req = rt.ServeRequest()
         req.module = module.handle
         req.inputs = rt.Inputs(len(rows), len(cols), rows.ctypes.data_as(C.POINTER(C.c_int64)),
@@ -50,6 +54,14 @@ class Server(object):
         req.tuning = C.pointer(tuning) if tuning is not None else None
         req.arrival_ns = int(arrival_ns)
         req.keep_result = 1 if keep else 0
+        if batch is not None:
+            bmod, bb = batch
+            req.batch_module = bmod.handle
+            req.batch_ncols, req.batch_ntables = len(bb.prefix), len(bb.tableMap)
+            req.batch_cols = C.cast(bb.cols, C.POINTER(C.c_void_p))
+            req.batch_col_prefix = C.cast(bb.prefix, C.POINTER(C.c_int32))
+            req.batch_table_map = C.cast(bb.tableMap, C.POINTER(C.c_int32))
+            self._alive.append(bb)
         qid = C.c_int64()
         rt.check(rt.lib().dqc_serve_submit(self.handle, C.byref(req), C.byref(qid)), "dqc_serve_submit")
         return qid.value
@@ -86,7 +98,7 @@ class PlanSet(object):
 
     def __init__(self, configs=tuple(CONFIG_PLANS), tuned=True):
         from dogqc_amd import autotune
-        self.modules, self.tunings, self.names = {}, {}, {}
+        self.modules, self.tunings, self.names, self.batchModules = {}, {}, {}, {}
         for cfg in configs:
             name = CONFIG_PLANS[cfg]
             rec = None
@@ -103,6 +115,8 @@ class PlanSet(object):
                 mod = rt.PlanModule(planlib.translate(name).compile(name))
                 tun = rt.newTuning()
             self.modules[cfg], self.tunings[cfg], self.names[cfg] = mod, tun, name
+            bname = "%s_batch" % cfg
+            self.batchModules[cfg] = rt.PlanModule(planlib.translate(bname).compile(bname))
 
 
 class Query(object):
@@ -130,6 +144,26 @@ def binding(module, tables, rows):
     for c in desc["columns"]:
         cols[c["index"]] = tables.store[(c["table"], c["attr"], c["part"])].ptr
     return r, cols
+
+
+class BatchBinding(object):
+    """How a config's batched module reads the resident tables
+    (dqc_serve_request.batch_*): column pointers, which columns hold the
+    members' prefixes (of which table), and which of the plain module's
+    tables each batched-module table takes its row count from."""
+
+    def __init__(self, plain, batched, tables):
+        pdesc, bdesc = plain.desc, batched.desc
+        ncols = max(1, len(bdesc["columns"]))
+        self.cols = (C.c_void_p * ncols)()
+        self.prefix = (C.c_int32 * ncols)(*([-1] * ncols))
+        for c in bdesc["columns"]:
+            if c["table"] == "__batch":
+                self.prefix[c["index"]] = bdesc["tables"].index(c["attr"])
+            else:
+                self.cols[c["index"]] = tables.store[(c["table"], c["attr"], c["part"])].ptr
+        self.tableMap = (C.c_int32 * len(bdesc["tables"]))(
+            *[-1 if t == "__batch" else pdesc["tables"].index(t) for t in bdesc["tables"]])
 
 
 def workspaceBytes(tables, plans, config, scale):
@@ -160,17 +194,25 @@ def poissonArrivals(queries, qps, seed=5):
 
 
 def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600, log=None, max_heavy=0,
-        heavy_bytes=0):
+        heavy_bytes=0, max_batch=0):
     """Run queries through a server with `slots` slots; returns (records,
     server) -- the server stays alive so kept results can be read.  With
-    `log`, reports every 5 s how many queries have finished."""
-    srv = Server(slots, arena_bytes, max_heavy=max_heavy, heavy_bytes=heavy_bytes)
+    `max_batch` > 1, waiting queries of one config run in batches of up to
+    that many (batched plans, dogqc_amd/batching.py).  With `log`, reports
+    every 5 s how many queries have finished."""
+    srv = Server(slots, arena_bytes, max_heavy=max_heavy, heavy_bytes=heavy_bytes, max_batch=max_batch)
     ids = []
+    bindings = {}
     for i, q in enumerate(queries):
         rows = prefixRows(tables, q.config, q.scale)
         r, cols = binding(plans.modules[q.config], tables, rows)
+        batch = None
+        if max_batch > 1:
+            if q.config not in bindings:
+                bindings[q.config] = BatchBinding(plans.modules[q.config], plans.batchModules[q.config], tables)
+            batch = (plans.batchModules[q.config], bindings[q.config])
         ids.append(srv.submit(plans.modules[q.config], r, cols, plans.tunings[q.config], q.arrival_ns,
-                              keep=i in keep))
+                              keep=i in keep, batch=batch))
     srv.start()
     deadline = time.perf_counter() + timeout_s
     while True:
@@ -209,6 +251,18 @@ def perConfig(queries, recs):
     return out
 
 
+def latencyByConfig(queries, recs):
+    """p50/p99 latency (arrival to completion, ms) and mean scale per config."""
+    out = OrderedDict()
+    for c in CONFIG_PLANS:
+        idx = [i for i, q in enumerate(queries) if q.config == c]
+        if idx:
+            lat = np.array([(recs[i].done_ns - recs[i].arrival_ns) * 1e-6 for i in idx])
+            out[c] = {"queries": len(idx), "p50_ms": float(np.percentile(lat, 50)),
+                      "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max())}
+    return out
+
+
 def latencyStats(recs):
     lat = np.array([(r.done_ns - r.arrival_ns) * 1e-6 for r in recs])
     first = min(r.arrival_ns for r in recs)
@@ -220,15 +274,23 @@ def latencyStats(recs):
 
 
 def resultOf(plans, config, rec):
-    """Result columns (and c3's second aggregation) of a kept record."""
+    """Result columns (and c3's second aggregation) of a kept record; a
+    query that ran in a batch gets its member's combination of the batch's
+    segments."""
     res = rt.QueryResult()
     res.ws, res.ws_bytes, res.layout, res.status = rec.ws, rec.ws_bytes, rec.layout, rec.status
-    mod = plans.modules[config]
+    batched = rec.batch_size > 1
+    mod = plans.batchModules[config] if batched else plans.modules[config]
     ex = rt.Execution(mod, None, None, res)
     ex.owned = False
     out = runner.resultColumns(mod, ex, None)
+    temps = {t["name"]: runner.tempColumns(mod, ex, None, t["name"]) for t in mod.desc["temps"]
+             if t["name"] == "c3_by_nation_year"}
+    if batched:
+        from dogqc_amd import batching
+        out, temps = batching.combine(config, out, temps, rec.batch_index)
     if config == "c3":
-        out = (out, runner.tempColumns(mod, ex, None, "c3_by_nation_year"))
+        out = (out, temps["c3_by_nation_year"])
     return out
 
 
@@ -294,7 +356,7 @@ def matchesOracle(config, got, want):
 
 
 def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None,
-              max_heavy=1, heavy_mb=64.0):
+              max_heavy=1, heavy_mb=64.0, max_batch=16):
     """Serving benchmark on resident tables at `scale` x the base sizes.
     Returns a dict of capacity (serial / batched QPS), latency at a fixed
     Poisson rate (load x the serial capacity) and verification results."""
@@ -316,7 +378,8 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     for c in CONFIG_PLANS:
         sample += [i for i in order if queries[i].config == c][:max(1, verify // len(CONFIG_PLANS))]
     out = {"scale": scale, "resident_gb": tables.nbytes() / 1e9, "queries": nqueries, "slots": slots,
-           "arena_gb": arena_gb, "max_heavy": max_heavy, "heavy_mb": heavy_mb, "plans": dict(plans.names)}
+           "arena_gb": arena_gb, "max_heavy": max_heavy, "heavy_mb": heavy_mb, "max_batch": max_batch,
+           "plans": dict(plans.names)}
     # capacity: every query arrives at t = 0
     for q in queries:
         q.arrival_ns = 0
@@ -335,10 +398,13 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
             list(L.mm_entries[:L.nhts]), L.result_capacity, list(L.temp_capacity[:L.ntemps]),
             list(L.split_capacity[:L.nhts])))
     say("serial capacity: %.1f q/s" % out["serial"]["qps"])
-    hv = {"max_heavy": max_heavy, "heavy_bytes": int(heavy_mb * (1 << 20))}
+    hv = {"max_heavy": max_heavy, "heavy_bytes": int(heavy_mb * (1 << 20)), "max_batch": max_batch}
     recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample), log=say, **hv)
     out["batched"] = latencyStats(recsB)
     out["batched_by_config"] = perConfig(queries, recsB)
+    sizes = [r.batch_size for r in recsB if r.batch_size > 1]
+    out["batched"]["in_batches"] = len(sizes)
+    out["batched"]["mean_batch"] = float(np.mean(sizes)) if sizes else 1.0
     say("batched capacity: %.1f q/s" % out["batched"]["qps"])
     # alone vs mixed, and the CPU oracle, on the sample
     alone_same, oracle_ok = 0, 0
@@ -359,9 +425,14 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     recs, srv = run(tables, plans, queries, 1, arena, log=say)
     srv.destroy()
     out["serial_at_rate"] = latencyStats(recs)
+    out["serial_at_rate_by_config"] = latencyByConfig(queries, recs)
     recs, srv = run(tables, plans, queries, slots, arena, log=say, **hv)
     srv.destroy()
     out["batched_at_rate"] = latencyStats(recs)
+    out["batched_at_rate_by_config"] = latencyByConfig(queries, recs)
+    sizes = [r.batch_size for r in recs if r.batch_size > 1]
+    out["batched_at_rate"]["in_batches"] = len(sizes)
+    out["batched_at_rate"]["mean_batch"] = float(np.mean(sizes)) if sizes else 1.0
     out["offered_qps"] = rate
     out["speedup"] = out["batched"]["qps"] / out["serial"]["qps"]
     say("at %.1f q/s: p99 serial %.2f ms, batched %.2f ms" % (rate, out["serial_at_rate"]["p99_ms"],

@@ -50,11 +50,14 @@ def test_datagen_header_and_exports():
 
 def test_serve_structs_match_header():
     # slots, max_retries, arena_bytes, bypass_ms, size_feedback, max_heavy, 4 bytes of
-    # padding before heavy_bytes, reserved[2]
+    # padding before heavy_bytes, max_batch, reserved
     assert ctypes.sizeof(rt.ServeConfig) == 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8 + 4 * 2 == 48
+    assert rt.ServeConfig.max_batch.offset == 40
     assert rt.ServeConfig.heavy_bytes.offset == 32
-    assert ctypes.sizeof(rt.ServeRequest) == 8 + 24 + 8 + 8 + 4 + 4 == 56
-    assert ctypes.sizeof(rt.ServeRecord) == 8 * 3 + 4 * 4 + 8 + 8 + ctypes.sizeof(rt.Layout) + ctypes.sizeof(rt.Status)
+    assert ctypes.sizeof(rt.ServeRequest) == 8 + 24 + 8 + 8 + 4 + 4 + 8 * 4 + 4 + 4 == 96
+    assert rt.ServeRequest.batch_module.offset == 56
+    assert ctypes.sizeof(rt.ServeRecord) == 8 * 3 + 4 * 4 + 8 + 8 + 4 + 4 + ctypes.sizeof(rt.Layout) + \
+        ctypes.sizeof(rt.Status)
 
 
 def test_plan_module_exports_and_describe():

@@ -155,3 +155,30 @@ def test_overflow_is_readmitted():
     got = serving.resultOf(plans, "c1", recs[0])
     assert serving.matchesOracle("c1", got, serving.oracleResult(T, "c1", 0.02))
     srv.destroy()
+
+
+def test_batched_queries_equal_alone_and_oracle():
+    """With max_batch, waiting queries of one config run as one batched plan
+    (all arrive at once here, so batches form); each member's combined result
+    equals its run alone and the CPU oracle, and kept batch workspaces are
+    released through their members."""
+    print("gpu case: serving batches", flush=True)
+    T = tables(0.02)
+    plans = serving.PlanSet()
+    queries = serving.makeMix(32, seed=12, scaleRange=(0.001, 0.02))
+    keep = set(range(len(queries)))
+    recsA, srvA = serving.run(T, plans, queries, 1, 4 << 30, keep=keep)
+    recsB, srvB = serving.run(T, plans, queries, 4, 4 << 30, keep=keep, max_batch=8)
+    assert sum(1 for r in recsB if r.batch_size > 1) >= 8
+    for i, q in enumerate(queries):
+        a = serving.resultOf(plans, q.config, recsA[i])
+        b = serving.resultOf(plans, q.config, recsB[i])
+        assert serving.sameResult(q.config, a, b), "query %d %s x%.4f alone != batched (batch %d)" % (
+            i, q.config, q.scale, recsB[i].batch_size)
+        assert serving.matchesOracle(q.config, b, serving.oracleResult(T, q.config, q.scale)), \
+            "query %d %s x%.4f batched != oracle" % (i, q.config, q.scale)
+    for i in range(len(queries)):
+        srvB.release(i)
+    srvA.destroy()
+    srvB.destroy()
+
