@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--max-heavy", type=int, default=1, help="c4: heavy queries running at once (0 = no limit)")
     ap.add_argument("--max-batch", type=int, default=16,
                     help="c4: waiting queries of one config run as one batched plan, up to this many (<= 1: off)")
+    ap.add_argument("--batch-ratio", type=int, default=4,
+                    help="c4: queries share a batch only if their driving rows are within this factor (0 = any)")
     ap.add_argument("--heavy-mb", type=float, default=64.0, help="c4: workspace size of a heavy query")
     ap.add_argument("--rows-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -235,7 +237,8 @@ def benchServing(args):
     nq = max(8, args.steps) if args.steps != 50 else 200
     log = lambda m: print("[c4] " + m, file=sys.stderr, flush=True)
     r = serving.benchmark(scale=args.scale, nqueries=nq, slots=args.slots, arena_gb=args.arena_gb, log=log,
-                          max_heavy=args.max_heavy, heavy_mb=args.heavy_mb, max_batch=args.max_batch)
+                          max_heavy=args.max_heavy, heavy_mb=args.heavy_mb, max_batch=args.max_batch,
+                          batch_ratio=args.batch_ratio)
     out = {
         "metric": METRIC,
         "value": r["batched"]["qps"],
@@ -251,7 +254,8 @@ def benchServing(args):
         "data": "synthetic (generated on the device, dogqc_amd/datagen.py)",
         "config": {"workload": SERVING["desc"], "resident_gb": r["resident_gb"], "scale": r["scale"],
                    "queries": nq, "slots": r["slots"], "arena_gb": r["arena_gb"], "max_heavy": r["max_heavy"],
-                   "heavy_mb": r["heavy_mb"], "max_batch": r["max_batch"], "plans": r["plans"],
+                   "heavy_mb": r["heavy_mb"], "max_batch": r["max_batch"], "batch_ratio": r["batch_ratio"],
+                   "plans": r["plans"],
                    "parallelism": "single GPU"},
         "roofline": None,
         "serial_qps": r["serial"]["qps"],

@@ -347,9 +347,15 @@ DQC_DEVICE_INLINE int64_t agg_slot_hashed(uint64_t* tags, uint32_t cap, uint64_t
 }
 
 // LDS variant of agg_slot_exact (workgroup-private table of `cap` slots).
+// Probes at most LDS_MAX_PROBE slots: a key either has a slot within that
+// window of its start or never gets one (slots are never freed), so all its
+// rows agree; a full table sends new keys to the global table after a short
+// probe instead of a scan of every slot.
+constexpr uint32_t LDS_MAX_PROBE = 64;
+
 DQC_DEVICE_INLINE int lds_slot_exact(uint64_t* ltags, uint32_t cap, uint64_t tag, uint64_t h) {
     uint32_t s = slot_of(h, cap);
-    for (uint32_t n = 0; n < cap; ++n) {
+    for (uint32_t n = 0; n < cap && n < LDS_MAX_PROBE; ++n) {
         uint64_t t = __hip_atomic_load(&ltags[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (t == tag) return (int)s;
         if (t == TAG_EMPTY) {
@@ -362,6 +368,101 @@ DQC_DEVICE_INLINE int lds_slot_exact(uint64_t* ltags, uint32_t cap, uint64_t tag
         s = next_slot(s, cap);
     }
     return -1;
+}
+
+// ------------------------------------------------- partitioned group-by
+// The first pass of a partitioned group-by gives workgroup w of `nseg` its
+// own segment of every partition b: `cap` records at (b * nseg + w) * cap,
+// filled through an LDS cursor, and stores the segment's record count at
+// counts[b * nseg + w] at its end.  The second pass gives workgroup b
+// partition b: part_segments puts the segments' exclusive prefix sums in
+// pref[0..nseg] (LDS) and returns the partition's records, which the rows
+// 0..total-1 of the second pass then address through part_record.
+
+// Exclusive scan over the workgroup (blockDim.x a multiple of 64, at most
+// 1024); sh holds blockDim.x / 64 + 1 words.  Every thread must call it.
+DQC_DEVICE_INLINE uint32_t wg_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const uint32_t w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    const uint32_t incl = (uint32_t)wave_incl_scan((uint64_t)v);
+    if (lane_id() == WAVE - 1) sh[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t i = 0; i < nw; ++i) {
+            const uint32_t x = sh[i];
+            sh[i] = run;
+            run += x;
+        }
+        sh[nw] = run;
+    }
+    __syncthreads();
+    const uint32_t r = sh[w] + incl - v;
+    total = sh[nw];
+    __syncthreads();
+    return r;
+}
+
+DQC_DEVICE_INLINE uint32_t part_segments(const uint32_t* counts, uint32_t part, uint32_t nseg, uint32_t cap,
+                                         uint32_t* pref, uint32_t* sh) {
+    const uint32_t k = (nseg + blockDim.x - 1) / blockDim.x;
+    const uint32_t first = threadIdx.x * k;
+    const uint32_t* c = counts + (uint64_t)part * nseg;
+    uint32_t s = 0;
+    for (uint32_t j = 0; j < k; ++j)
+        if (first + j < nseg) s += std::min<uint32_t>(c[first + j], cap);
+    uint32_t total = 0;
+    uint32_t ex = wg_excl_scan(s, sh, total);
+    for (uint32_t j = 0; j < k; ++j) {
+        if (first + j < nseg) {
+            pref[first + j] = ex;
+            ex += std::min<uint32_t>(c[first + j], cap);
+        }
+    }
+    if (threadIdx.x == 0) pref[nseg] = total;
+    __syncthreads();
+    return total;
+}
+
+// Records of the thread's U rows row[u] < pref[nseg] of partition `part`:
+// the segment lo with pref[lo] <= row < pref[lo + 1], at offset
+// row - pref[lo].  The U binary searches advance in lockstep (the same
+// number of halvings for every row), so their LDS reads overlap.
+template <int U>
+DQC_DEVICE_INLINE void part_records(const uint32_t* pref, uint32_t nseg, uint32_t part, uint32_t cap,
+                                    const int64_t (&row)[U], const int (&act)[U], uint64_t (&rec)[U]) {
+    uint32_t lo[U], hi[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        lo[u] = 0u;
+        hi[u] = act[u] ? nseg : 1u;
+    }
+    for (uint32_t n = nseg; n > 1u; n = (n + 1u) >> 1) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (hi[u] - lo[u] > 1u) {
+                const uint32_t mid = (lo[u] + hi[u]) >> 1;
+                if (pref[mid] <= (uint32_t)row[u]) lo[u] = mid;
+                else hi[u] = mid;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        rec[u] = act[u] ? ((uint64_t)part * nseg + lo[u]) * cap + ((uint32_t)row[u] - pref[lo[u]]) : 0ull;
+}
+
+// A value in one 8-byte word of a record, and back.
+template <typename T>
+DQC_DEVICE_INLINE uint64_t to_word(T v) {
+    uint64_t w = 0;
+    __builtin_memcpy(&w, &v, sizeof(T));
+    return w;
+}
+template <typename T>
+DQC_DEVICE_INLINE T from_word(uint64_t w) {
+    T v;
+    __builtin_memcpy(&v, &w, sizeof(T));
+    return v;
 }
 
 // Orders this wavefront's earlier LDS writes before its later LDS accesses

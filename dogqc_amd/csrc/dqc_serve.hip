@@ -439,10 +439,17 @@ struct Server {
                 // waiting queries of the same plan join this one in a batch
                 std::vector<int64_t> group{*it};
                 const int maxb = std::min(cfg.max_batch, DQC_SERVE_MAX_BATCH);
+                const double head = (double)drivingRows(r);
+                auto similar = [this, head](const Request& o) {
+                    if (cfg.batch_ratio <= 0) return true;
+                    const double x = (double)drivingRows(o), q = (double)cfg.batch_ratio;
+                    return x <= head * q && head <= x * q;
+                };
                 for (auto jt = std::next(it); jt != pending.end() && (int)group.size() < maxb; ++jt) {
                     const Request& o = *reqs[*jt];
                     if (o.rec.arrival_ns > ta) break;
-                    if (!o.internal && o.module == r.module && o.batchModule == r.batchModule) group.push_back(*jt);
+                    if (!o.internal && o.module == r.module && o.batchModule == r.batchModule && similar(o))
+                        group.push_back(*jt);
                 }
                 const int64_t bid = group.size() >= 2 ? makeBatch(group) : -1;
                 if (bid >= 0) {

@@ -44,7 +44,7 @@ class Tuning(C.Structure):
                 ("mm_entries", C.c_int64 * MAX_HTS), ("result_capacity", C.c_int64),
                 ("temp_capacity", C.c_int64 * MAX_TEMPS), ("split_capacity", C.c_int64 * MAX_HTS),
                 ("split_threshold", C.c_int32 * MAX_PIPES), ("profile", C.c_int32), ("xcd_replicas", C.c_int32),
-                ("census", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("census", C.c_int32), ("part_agg", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Status(C.Structure):
@@ -71,7 +71,7 @@ class QueryResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_retries", C.c_int32), ("arena_bytes", C.c_uint64),
                 ("bypass_ms", C.c_int32), ("size_feedback", C.c_int32), ("max_heavy", C.c_int32),
-                ("heavy_bytes", C.c_uint64), ("max_batch", C.c_int32), ("reserved", C.c_int32)]
+                ("heavy_bytes", C.c_uint64), ("max_batch", C.c_int32), ("batch_ratio", C.c_int32)]
 
 
 class ServeRequest(C.Structure):

@@ -50,9 +50,9 @@ def test_datagen_header_and_exports():
 
 def test_serve_structs_match_header():
     # slots, max_retries, arena_bytes, bypass_ms, size_feedback, max_heavy, 4 bytes of
-    # padding before heavy_bytes, max_batch, reserved
+    # padding before heavy_bytes, max_batch, batch_ratio
     assert ctypes.sizeof(rt.ServeConfig) == 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8 + 4 * 2 == 48
-    assert rt.ServeConfig.max_batch.offset == 40
+    assert rt.ServeConfig.max_batch.offset == 40 and rt.ServeConfig.batch_ratio.offset == 44
     assert rt.ServeConfig.heavy_bytes.offset == 32
     assert ctypes.sizeof(rt.ServeRequest) == 8 + 24 + 8 + 8 + 4 + 4 + 8 * 4 + 4 + 4 == 96
     assert rt.ServeRequest.batch_module.offset == 56
@@ -79,6 +79,7 @@ def test_struct_sizes_match_header():
     # (the same sizes are static_asserts in the runtime, which includes the header)
     assert ctypes.sizeof(rt.Tuning) == 4 * 32 * 4 + 4 * 32 + 8 * 32 * 2 + 8 + 8 * 8 + 8 * 32 + 4 * 32 + 4 + 4 * 7 \
         == 1640
+    assert rt.Tuning.part_agg.offset == rt.Tuning.census.offset + 4
     assert ctypes.sizeof(rt.Status) == 16 + 8 + 8 * 32 + 8 * 8 + 8 * 32 + 8 * 32 == 856
     assert ctypes.sizeof(rt.Layout) == 8 + 4 + 4 + 8 * 32 * 2 + 8 + 4 + 4 + 8 * 64 + 8 + 4 + 4 + 8 * 8 + 8 * 64 \
         + 8 * 32 == 1904

@@ -447,3 +447,57 @@ def test_batched_plan_on_gpu(config):
         for name, cols in tmp.items():
             w = {c: wantTemps[name][c] for c in cols}
             assertSameResult(cols, w, {"revenue"}, label="%s member %d %s" % (config, k, name))
+
+
+# ------------------------------------------------------------ partitioned group-by
+
+# plans with a group-by the translator can launch partitioned (exact keys, a
+# pipeline loop uniform over the workgroup)
+PART_FEATURES = ["f_semi", "f_anti_multi", "f_anti_neg", "f_cross", "f_like_patterns"]
+PART_TPCH = [1, 2, 8, 11, 15, 18, 20]
+
+
+def partTuning(parts):
+    t = rt.newTuning()
+    t.part_agg = parts
+    return t
+
+
+def assertPartitioned(name):
+    ann = [a for p in module(name).desc["pipelines"] for a in p["annotations"]]
+    assert any(a.startswith("PART") for a in ann), name
+
+
+@pytest.mark.parametrize("parts", [2, 64])
+@pytest.mark.parametrize("name", PART_FEATURES)
+def test_partitioned_groupby_features(name, parts):
+    """Group-bys forced through the partitioned tier (tuning part_agg = n
+    partitions): rows scattered to partitions, each partition aggregated by
+    one workgroup in LDS."""
+    assertPartitioned(name)
+    for size in [(5000, 700), (200, 50), (1, 1)]:
+        check(name, planlib.featureData(*size), tuning=partTuning(parts),
+              label="parts=%d t=%d u=%d" % ((parts,) + size))
+
+
+@pytest.mark.parametrize("parts", [2, 64, 1024])
+@pytest.mark.parametrize("q", PART_TPCH)
+def test_partitioned_groupby_tpch(q, parts):
+    """TPC-H group-bys through the partitioned tier.  Q1's 4 groups land in at
+    most 4 partitions, far more rows than a partition holds: the rows beyond
+    its capacity update the global table directly."""
+    name = "tpch%d" % q
+    assertPartitioned(name)
+    check(name, tpchAcc(0.05), tuning=partTuning(parts), storeKey=("tpch", 0.05), label="parts=%d" % parts)
+
+
+@pytest.mark.parametrize("parts", [-1, 0, 16, 64])
+def test_partitioned_groupby_c3(parts):
+    """c3's by-item group-by (numgroups 120000: partitioned by default, 64
+    partitions), off (-1) and forced to 16 / 64 partitions."""
+    print("gpu case: c3_refill part_agg=%d" % parts, flush=True)
+    acc = workloadAcc("c3", {"sales": 200000, "item": 20000, "cust": 2000, "nation": 25}, seed=33)
+    got, ny, _ = c3Run(acc, tuning=partTuning(parts), plan="c3_refill")
+    want, wantNy = c3Oracle(acc)
+    assertSameResult(got, want, {"revenue"}, label="c3 by item parts=%d" % parts)
+    assertSameResult(ny, wantNy, {"revenue"}, label="c3 by nation, year parts=%d" % parts)

@@ -182,3 +182,32 @@ def test_batched_queries_equal_alone_and_oracle():
     srvA.destroy()
     srvB.destroy()
 
+
+
+def test_batch_ratio_keeps_batches_of_similar_size():
+    """With batch_ratio, a batch holds only queries whose driving row counts
+    are within that factor of its first query's; results stay the same."""
+    print("gpu case: serving batch ratio", flush=True)
+    T = tables(0.02)
+    plans = serving.PlanSet()
+    queries = serving.makeMix(32, seed=13, scaleRange=(0.0005, 0.02))
+    keep = set(range(len(queries)))
+    recsA, srvA = serving.run(T, plans, queries, 1, 4 << 30, keep=keep)
+    recsB, srvB = serving.run(T, plans, queries, 4, 4 << 30, keep=keep, max_batch=8, batch_ratio=2)
+    driving = [max(serving.prefixRows(T, q.config, q.scale).values()) for q in queries]
+    batches = {}
+    for i, r in enumerate(recsB):
+        if r.batch_size > 1:
+            batches.setdefault((r.slot, r.admit_ns), []).append(i)
+    assert batches, "no batch formed"
+    for members in batches.values():
+        assert len({queries[i].config for i in members}) == 1
+        rows = [driving[i] for i in members]
+        assert max(rows) <= 4 * min(rows), [(queries[i].config, queries[i].scale) for i in members]
+    for i, q in enumerate(queries):
+        assert serving.sameResult(q.config, serving.resultOf(plans, q.config, recsA[i]),
+                                  serving.resultOf(plans, q.config, recsB[i])), "query %d alone != batched" % i
+    for i in range(len(queries)):
+        srvB.release(i)
+    srvA.destroy()
+    srvB.destroy()
