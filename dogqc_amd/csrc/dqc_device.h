@@ -370,6 +370,24 @@ DQC_DEVICE_INLINE int lds_slot_exact(uint64_t* ltags, uint32_t cap, uint64_t tag
     return -1;
 }
 
+// ------------------------------------------------------ join filter
+// Blocked Bloom filter of a unique join's build keys: `words` 32-bit words
+// (a power of two), two bits of one word per key, all from the key's hash.
+// The build sets them; a probe pipeline copies the filter into LDS and skips
+// the hash-table probe of rows whose bits are not both set (no match is
+// possible), so only matches and false positives reach the table.
+DQC_DEVICE_INLINE uint32_t bloom_word(uint64_t h, uint32_t words) { return (uint32_t)h & (words - 1u); }
+DQC_DEVICE_INLINE uint32_t bloom_mask(uint64_t h) {
+    return (1u << ((uint32_t)(h >> 32) & 31u)) | (1u << ((uint32_t)(h >> 40) & 31u));
+}
+DQC_DEVICE_INLINE void bloom_set(uint32_t* f, uint32_t words, uint64_t h) {
+    atomicOr(&f[bloom_word(h, words)], bloom_mask(h));
+}
+DQC_DEVICE_INLINE bool bloom_test(const uint32_t* f, uint32_t words, uint64_t h) {
+    const uint32_t m = bloom_mask(h);
+    return (f[bloom_word(h, words)] & m) == m;
+}
+
 // ------------------------------------------------- partitioned group-by
 // The first pass of a partitioned group-by gives workgroup w of `nseg` its
 // own segment of every partition b: `cap` records at (b * nseg + w) * cap,

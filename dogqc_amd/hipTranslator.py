@@ -147,6 +147,7 @@ class Pipeline(object):
         self.phaseBetween = None     # launch steps between PHASE 0 and PHASE 1
         self.phaseRows = {}          # phase -> (host grid rows, device row count) overrides
         self.partAgg = None          # partitioned group-by: PHASE 1 gives workgroup b partition b
+        self.ldsFilter = None        # join filter at the start of dynamic LDS: {"dev", "host"} byte counts
         self.dynLdsExpr = None
         self.refillPoints = []
         self.annotations = []
@@ -513,6 +514,8 @@ class Program(object):
         h.emit("    uint32_t apbits[%d];" % max(1, len(self.hts)))
         h.emit("    int64_t apcap[%d];" % max(1, len(self.hts)))
         h.emit("    uint32_t apseg[%d];" % max(1, len(self.hts)))
+        h.emit("    uint32_t bfw[%d];" % max(1, len(self.hts)))
+        h.emit("    uint32_t bfp[%d];" % max(1, len(self.pipes)))
         h.emit("    int64_t apgroups[%d];" % max(1, len(self.hts)))
         h.emit("};")
         h.emit("static inline uint64_t align256(uint64_t x) { return (x + 255u) & ~(uint64_t)255u; }")
@@ -556,11 +559,13 @@ class Program(object):
         for opId, expr in self.estimates.items():
             h.emit("const int64_t est_%d = std::max<int64_t>(0, %s);" % (opId, expr))
             h.emit("(void)est_%d;" % opId)
+        h.emit("for (int i = 0; i < %d; ++i) L.bfp[i] = 0u;" % max(1, len(self.pipes)))
         h.emit("for (int i = 0; i < %d; ++i) { L.arep[i] = 1u; L.apart[i] = 0u; L.apbits[i] = 0u; L.apcap[i] = 0; "
-               "L.apseg[i] = 0u; L.apgroups[i] = 0; }"
+               "L.apseg[i] = 0u; L.apgroups[i] = 0; L.bfw[i] = 0u; }"
                % max(1, len(self.hts)))
         for name, stmt in self.hostVars.items():
-            h.emit(stmt)
+            # a callable is rendered now, when every pipeline is complete
+            h.emit(stmt() if callable(stmt) else stmt)
         h.emit("for (int i = 0; i < %d; ++i) L.scap[i] = 0;" % max(1, len(self.hts)))
         for name, stmt in self.splitVars.items():
             h.emit(stmt)
@@ -718,7 +723,8 @@ def pipelineLaunch(prog, pipe, phase=None, rowsExpr=None, perCu=8):
         # rows are scattered to partitions: no LDS group table in this pass
         lines.append("const bool part = %s > 0u;" % pa["hostNp"])
         lines.append("if (part) kp.lw = 0u;")
-        lds = "(part ? (size_t)0 : %s)" % lds
+        keep = "(size_t)0" if pipe.ldsFilter is None else "(size_t)(%s)" % pipe.ldsFilter["host"]
+        lds = "(part ? %s : %s)" % (keep, lds)
     lines.append("const unsigned g = gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, %s);"
                  % (pipe.pipeId, pipe.pipeId, pipe.defaultGrid, rows, pipe.bs * pipe.U, per))
     if pa is not None and phase == 0:
@@ -1812,6 +1818,12 @@ class AggregationTranslator(UnaryTranslator):
         replicated R times; flushed once per workgroup."""
         pid = pipe.pipeId
         accBytes = sum(CType.size[acc["ctype"]] for acc in self.accs)
+        # LDS in use besides the table: static arrays, a join filter in front
+        used = "(uint64_t)%du" % (staticLdsBytes(pipe) + 1024)
+        devBase, hostBase = "(uint64_t)0", None
+        if pipe.ldsFilter is not None:
+            used = "(%s + %s)" % (used, pipe.ldsFilter["host"])
+            devBase, hostBase = pipe.ldsFilter["dev"], pipe.ldsFilter["host"]
         prog.hostVars["lds%d" % pid] = (
             "{ int64_t w = t->lds_slots[%d]; uint32_t W = 0, R = 1;\n"
             "  if (w > 0) W = pow2floor((uint64_t)w);\n"
@@ -1821,12 +1833,11 @@ class AggregationTranslator(UnaryTranslator):
             "    int64_t r = t->lds_replicas[%d];\n"
             "    if (r > 0) R = std::min<uint32_t>(64u, pow2floor((uint64_t)r));\n"
             "    else { R = 64u; while (R > 1u && (uint64_t)W * (8u + (uint64_t)R * %du) > %du) R /= 2u; }\n"
-            "    while (W > 16u && (uint64_t)W * (8u + (uint64_t)R * %du) + %du > maxLds()) W /= 2u;\n"
-            "    if ((uint64_t)W * (8u + (uint64_t)R * %du) + %du > maxLds()) W = 0;\n"
+            "    while (W > 16u && (uint64_t)W * (8u + (uint64_t)R * %du) + %s > maxLds()) W /= 2u;\n"
+            "    if ((uint64_t)W * (8u + (uint64_t)R * %du) + %s > maxLds()) W = 0;\n"
             "  }\n"
             "  L.ldsw[%d] = W; L.ldsr[%d] = W ? R : 1u; }"
-            % (pid, est, pid, accBytes, LDS_AGG_BUDGET, accBytes, staticLdsBytes(pipe) + 1024,
-               accBytes, staticLdsBytes(pipe) + 1024, pid, pid))
+            % (pid, est, pid, accBytes, LDS_AGG_BUDGET, accBytes, used, accBytes, used, pid, pid))
         pipe.param("lw", "uint32_t", "L.ldsw[%d]" % pid)
         pipe.param("lr", "uint32_t", "L.ldsr[%d]" % pid)
         # dynamic LDS: tags, then one region per accumulator (8-byte aligned)
@@ -1834,10 +1845,12 @@ class AggregationTranslator(UnaryTranslator):
         for acc in self.accs:
             parts.append("(((uint64_t)L.ldsw[%d] * L.ldsr[%d] * %d + 7u) & ~(uint64_t)7u)"
                          % (pid, pid, CType.size[acc["ctype"]]))
+        if hostBase is not None:
+            parts.insert(0, hostBase)
         pipe.dynLdsExpr = " + ".join(parts)
         pro = pipe.prologue
-        pro.emit("uint64_t* ltags = (uint64_t*)dqc_dyn_lds;")
-        pro.emit("uint64_t loff = (uint64_t)p.lw * 8u;")
+        pro.emit("uint64_t* ltags = (uint64_t*)(dqc_dyn_lds + %s);" % devBase)
+        pro.emit("uint64_t loff = %s + (uint64_t)p.lw * 8u;" % devBase)
         for acc in self.accs:
             pro.emit("%s* lacc%d = (%s*)(dqc_dyn_lds + loff);" % (acc["ctype"], acc["i"], acc["ctype"]))
             pro.emit("loff += ((uint64_t)p.lw * p.lr * %d + 7u) & ~(uint64_t)7u;" % CType.size[acc["ctype"]])

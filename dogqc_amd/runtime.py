@@ -44,7 +44,8 @@ class Tuning(C.Structure):
                 ("mm_entries", C.c_int64 * MAX_HTS), ("result_capacity", C.c_int64),
                 ("temp_capacity", C.c_int64 * MAX_TEMPS), ("split_capacity", C.c_int64 * MAX_HTS),
                 ("split_threshold", C.c_int32 * MAX_PIPES), ("profile", C.c_int32), ("xcd_replicas", C.c_int32),
-                ("census", C.c_int32), ("part_agg", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("census", C.c_int32), ("part_agg", C.c_int32), ("join_filter", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class Status(C.Structure):

@@ -501,3 +501,36 @@ def test_partitioned_groupby_c3(parts):
     want, wantNy = c3Oracle(acc)
     assertSameResult(got, want, {"revenue"}, label="c3 by item parts=%d" % parts)
     assertSameResult(ny, wantNy, {"revenue"}, label="c3 by nation, year parts=%d" % parts)
+
+
+# ------------------------------------------------------------ join filters
+
+def filterTuning(kb):
+    t = rt.newTuning()
+    t.join_filter = kb
+    return t
+
+
+@pytest.mark.parametrize("kb", [-1, 1, 4, 64])
+@pytest.mark.parametrize("plan", ["c1", "c1_refill"])
+def test_join_filter_c1(plan, kb):
+    """The LDS filter of c1's selective dim join (tuning join_filter): off
+    (-1), 1 KB (many false positives), 4 KB and 64 KB; rows the filter rules
+    out skip the probe."""
+    for rows in [{"dim": 30000, "fact": 200000}, {"dim": 1, "fact": 1000}, {"dim": 1000, "fact": 0}]:
+        check(plan, workloadAcc("c1", rows, seed=51), tuning=filterTuning(kb), label="kb=%d %r" % (kb, rows))
+
+
+@pytest.mark.parametrize("kb", [-1, 1, 32, 96])
+def test_join_filter_c3(kb):
+    print("gpu case: c3_refill join_filter=%d" % kb, flush=True)
+    acc = workloadAcc("c3", {"sales": 200000, "item": 20000, "cust": 2000, "nation": 25}, seed=52)
+    got, ny, _ = c3Run(acc, tuning=filterTuning(kb), plan="c3_refill")
+    want, wantNy = c3Oracle(acc)
+    assertSameResult(got, want, {"revenue"}, label="c3 by item kb=%d" % kb)
+    assertSameResult(ny, wantNy, {"revenue"}, label="c3 by nation, year kb=%d" % kb)
+
+
+@pytest.mark.parametrize("kb", [-1, 1, 64])
+def test_join_filter_tpch10(kb):
+    check("tpch10", tpchAcc(0.05), tuning=filterTuning(kb), storeKey=("tpch", 0.05), label="kb=%d" % kb)
