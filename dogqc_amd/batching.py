@@ -94,11 +94,11 @@ def plan(config, acc):
                        alg.scan("c3_star"))
         ny = alg.createtemp("c3_by_nation_year", alg.aggregation(["yseg", "n_key"], aggs, yseg))
         byItem = alg.aggregation(["seg", "i_key"], aggs, alg.scan("c3_star"))
-        # (nation, year) groups stay on the LDS tier (2000 -> 4096 slots);
+        # (nation, year) groups: 175 per member, on the LDS tier (4096 slots);
         # (segment, item) groups are bounded by the star rows (no hint: sized
         # from the temp table's row estimate, so large batches do not retry)
         cfg = {j1.opId: {"multimatch": False, "selectivity": 0.06}, j2.opId: {"multimatch": False},
-               j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 2000}}
+               j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 176 * MEMBERS}}
         return alg, [star, ny, byItem], cfg, {"buffers": [j1.opId]}
     raise KeyError(config)
 

@@ -1828,7 +1828,8 @@ class AggregationTranslator(UnaryTranslator):
             "{ int64_t w = t->lds_slots[%d]; uint32_t W = 0, R = 1;\n"
             "  if (w > 0) W = pow2floor((uint64_t)w);\n"
             "  else if (w == 0) W = std::max<uint32_t>(16u, pow2ceil((uint64_t)std::max<int64_t>(1, %s) * 2u));\n"
-            "  if (W > 4096u) W = 0;\n"
+            "  if (W > 8192u) W = 0;\n"
+            "  W = std::min<uint32_t>(W, 4096u);\n"
             "  if (W) {\n"
             "    int64_t r = t->lds_replicas[%d];\n"
             "    if (r > 0) R = std::min<uint32_t>(64u, pow2floor((uint64_t)r));\n"

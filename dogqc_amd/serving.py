@@ -414,6 +414,12 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample), log=say, **hv)
     out["batched"] = latencyStats(recsB)
     out["batched_by_config"] = perConfig(queries, recsB)
+    for i in sorted(range(nqueries), key=lambda i: -recsB[i].retries)[:4]:
+        if recsB[i].retries:
+            L = recsB[i].layout
+            say("batched run retried: %s x%.2f (batch of %d) %d times: ht %s temp %s result %d" % (
+                queries[i].config, queries[i].scale, recsB[i].batch_size, recsB[i].retries,
+                list(L.ht_capacity[:L.nhts]), list(L.temp_capacity[:L.ntemps]), L.result_capacity))
     sizes = [r.batch_size for r in recsB if r.batch_size > 1]
     out["batched"]["in_batches"] = len(sizes)
     out["batched"]["mean_batch"] = float(np.mean(sizes)) if sizes else 1.0
