@@ -483,6 +483,14 @@ DQC_DEVICE_INLINE T from_word(uint64_t w) {
     return v;
 }
 
+// Compare-and-swap on an LDS word; returns the old value.
+template <typename Tag>
+DQC_DEVICE_INLINE Tag lds_cas(Tag* p, Tag expected, Tag desired) {
+    __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return expected;
+}
+
 // Orders this wavefront's earlier LDS writes before its later LDS accesses
 // by other lanes of the same wavefront (a wavefront's LDS operations complete
 // in issue order; the fences keep the compiler from reordering across).
@@ -730,6 +738,39 @@ __global__ void __launch_bounds__(256) dqc_census(const Tag* tags, uint64_t n, u
         c += tags[i] != (Tag)0 ? 1u : 0u;
     c = wave_sum(c);
     if (lane_id() == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+// --------------------------------------------- partitioned unique build
+// Keys whose probe run in the second pass of a partitioned build crossed
+// the end of their partition's slot range wait in a spill list of records
+// [tag, home slot, payload words...]; after that pass this kernel inserts
+// them with the ordinary CAS probe (every slot from the home slot to the
+// range end is taken, so the key lands where a sequential build would put
+// it) and copies each payload word's low `size[c]` bytes to column c.
+struct spill_cols {
+    void* ptr[8];
+    uint32_t size[8];
+    uint32_t n;
+};
+
+template <typename Tag>
+__global__ void __launch_bounds__(256) dqc_spill_insert(Tag* tags, uint32_t cap, const uint64_t* recs, uint32_t words,
+                                                          const uint32_t* count, uint32_t rec_cap, spill_cols cols,
+                                                          uint32_t* flags, uint32_t* overflow_mask,
+                                                          uint32_t overflow_bit) {
+    const uint32_t n = std::min<uint32_t>(*count, rec_cap);
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint64_t* r = recs + (uint64_t)i * words;
+        const Tag tag = (Tag)r[0];
+        const int64_t s = ht_insert_from<Tag>(tags, cap, tag, (uint32_t)r[1], 0u, flags);
+        if (s < 0) {
+            raise_status(flags, ST_HT_OVERFLOW);
+            atomicOr(overflow_mask, overflow_bit);
+            continue;
+        }
+        for (uint32_t c = 0; c < cols.n && c < 8u; ++c)
+            __builtin_memcpy((char*)cols.ptr[c] + (uint64_t)s * cols.size[c], &r[2 + c], cols.size[c]);
+    }
 }
 
 // ---------------------------------------------------- multimap scan kernels

@@ -45,6 +45,7 @@ from dogqc_amd.hiplang import CType, PACK_BITS
 class EquiJoinTranslator(BinaryTranslator):
 
     usePushDownJoin = False
+    usePartitionedBuild = False      # unique builds that can be partitioned are translated for it
     SPLIT_CHUNK = 64                 # matches per queued chunk
     DEFAULT_SPLIT_THRESHOLD = 1024   # list length above which a probe row is queued
     # unique inner joins at most this selective can take an LDS filter of
@@ -194,41 +195,14 @@ class EquiJoinTranslator(BinaryTranslator):
             fbf = pipe.param("jbf%d" % J, "uint32_t*", "B_" + bf)
             fbfw = pipe.param("jbfw%d" % J, "uint32_t", "L.bfw[%d]" % idx)
         if kind == "unique":
-            # first CAS of every row of the thread first, collisions after
             tt = self.tagType
-            jtag = pipe.reg("jtag%d" % J, tt)
-            jc = pipe.reg("jc%d" % J, "uint32_t")
-            jprev = pipe.reg("jprev%d" % J, tt)
-            with pipe.rowLoop():
-                pipe.code.open("if (active[u])")
-                self._hashCode(pipe, self.buildKeys, jtag, "jh%d" % J, tagIsReg=True)
-                if self.tag32:
-                    # tag 0: the extra slot at index cap (duplicates of that key are dropped)
-                    pipe.emit("%s = %s != 0u ? dqc::slot_of(jh%d, %s) : %s;" % (jc, jtag, J, fcap, fcap))
-                    pipe.emit("%s = dqc::cas_tag(&%s[%s], (%s)0, %s != 0u ? %s : 1u);"
-                              % (jprev, ftags, jc, tt, jtag, jtag))
-                else:
-                    pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jc, J, fcap))
-                    pipe.emit("%s = dqc::cas_tag(&%s[%s], (%s)0, %s);" % (jprev, ftags, jc, tt, jtag))
-                if fbf is not None:
-                    pipe.emit("if (%s) dqc::bloom_set(%s, %s, jh%d);" % (fbfw, fbf, fbfw, J))
-                pipe.code.close()
-            with pipe.rowLoop():
-                pipe.code.open("if (active[u])")
-                if self.tag32:
-                    pipe.emit("const int64_t js = %s == 0u ? (int64_t)%s : %s == 0u ? (int64_t)-2 : "
-                              "dqc::ht_insert_from<uint32_t>(%s, %s, %s, dqc::next_slot(%s, %s), 1u, &st->flags);"
-                              % (jprev, jc, jtag, ftags, fcap, jtag, jc, fcap))
-                else:
-                    pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : dqc::ht_insert_from<uint64_t>(%s, %s, %s, "
-                              "dqc::next_slot(%s, %s), 1u, &st->flags);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
-                pipe.code.open("if (js >= 0)")
-                for f, att in zip(fields, self.stored):
-                    pipe.emit("%s[js] = %s;" % (f, pipe.access(att)))
-                pipe.code.close()
-                pipe.emit("else if (js == -1) { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
-                          "atomicOr(&st->ht_overflow_mask, %du); }" % (1 << idx))
-                pipe.code.close()
+            ins = {"jtag": pipe.reg("jtag%d" % J, tt), "jc": pipe.reg("jc%d" % J, "uint32_t"),
+                   "jprev": pipe.reg("jprev%d" % J, tt), "ftags": ftags, "fcap": fcap, "fields": fields,
+                   "bloom": (fbf, fbfw) if fbf is not None else None}
+            if self._buildPartitionable(pipe):
+                self._partitionedBuild(ctxt, pipe, prog, ins, est)
+            else:
+                self._directInsert(pipe, ins, "active[u]")
             return
         # multimap: count pass (PHASE 0) and insert pass (PHASE 1) of one kernel
         pipe.phased = True
@@ -249,6 +223,267 @@ class EquiJoinTranslator(BinaryTranslator):
             self._storeEntry(pipe, idx, fecap, fields, "jpos")
             pipe.code.close()
             pipe.code.close()
+
+    def _firstCas(self, pipe, ins):
+        """Home slot and first CAS of a row whose tag and hash jhJ are set."""
+        J = self.algExpr.opId
+        jtag, jc, jprev, ftags, fcap, tt = ins["jtag"], ins["jc"], ins["jprev"], ins["ftags"], ins["fcap"], self.tagType
+        if self.tag32:
+            # tag 0: the extra slot at index cap (duplicates of that key are dropped)
+            pipe.emit("%s = %s != 0u ? dqc::slot_of(jh%d, %s) : %s;" % (jc, jtag, J, fcap, fcap))
+            pipe.emit("%s = dqc::cas_tag(&%s[%s], (%s)0, %s != 0u ? %s : 1u);" % (jprev, ftags, jc, tt, jtag, jtag))
+        else:
+            pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jc, J, fcap))
+            pipe.emit("%s = dqc::cas_tag(&%s[%s], (%s)0, %s);" % (jprev, ftags, jc, tt, jtag))
+
+    def _resolveInsert(self, pipe, ins, cond, values):
+        """After the first CAS: collisions, then the payload columns."""
+        idx = self.ht["index"]
+        jtag, jc, jprev, ftags, fcap = ins["jtag"], ins["jc"], ins["jprev"], ins["ftags"], ins["fcap"]
+        with pipe.rowLoop():
+            pipe.code.open("if (%s)" % cond)
+            if self.tag32:
+                pipe.emit("const int64_t js = %s == 0u ? (int64_t)%s : %s == 0u ? (int64_t)-2 : "
+                          "dqc::ht_insert_from<uint32_t>(%s, %s, %s, dqc::next_slot(%s, %s), 1u, &st->flags);"
+                          % (jprev, jc, jtag, ftags, fcap, jtag, jc, fcap))
+            else:
+                pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : dqc::ht_insert_from<uint64_t>(%s, %s, %s, "
+                          "dqc::next_slot(%s, %s), 1u, &st->flags);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
+            pipe.code.open("if (js >= 0)")
+            for f, v in zip(ins["fields"], values):
+                pipe.emit("%s[js] = %s;" % (f, v))
+            pipe.code.close()
+            pipe.emit("else if (js == -1) { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
+                      "atomicOr(&st->ht_overflow_mask, %du); }" % (1 << idx))
+            pipe.code.close()
+
+    def _directInsert(self, pipe, ins, cond):
+        """First CAS of every row of the thread first, collisions after."""
+        J = self.algExpr.opId
+        with pipe.rowLoop():
+            pipe.code.open("if (%s)" % cond)
+            self._hashCode(pipe, self.buildKeys, ins["jtag"], "jh%d" % J, tagIsReg=True)
+            self._firstCas(pipe, ins)
+            if ins["bloom"] is not None:
+                fbf, fbfw = ins["bloom"]
+                pipe.emit("if (%s) dqc::bloom_set(%s, %s, jh%d);" % (fbfw, fbf, fbfw, J))
+            pipe.code.close()
+        self._resolveInsert(pipe, ins, cond, [pipe.access(att) for att in self.stored])
+
+    # partitioned build (tuning part_build = n partitions; off by default:
+    # measured slower than the CAS build on c1's 300 k-key dim table, 45 vs
+    # 25 us, and even at 3 M keys, 222 vs 213 us; faster on c3's 2 M-row
+    # customer build at 10x, 143 vs 250 us): the LDS image of a partition's
+    # slot range may take at most this many bytes
+    PART_BUILD_SPAN_MAX = 49152
+
+    def _buildPartitionable(self, pipe):
+        """A unique build can be launched in two phases when its insert is not
+        nested in a loop of the pipeline, the keys are exact (the hash follows
+        from the tag) and at most 8 columns are stored, each in at most 8
+        bytes (a record word; not strings)."""
+        if not EquiJoinTranslator.usePartitionedBuild:
+            return False
+        if pipe.innerDepth or pipe.loopOps or pipe.phased or pipe.refillPoints:
+            return False
+        if any(l is None for l in pipe.code.lines):
+            return False
+        if not all(CType.size.get(pipe.attrs[att.id].ctype, 16) <= 8 for att in self.stored):
+            return False
+        return self.exact and len(self.stored) <= 8
+
+    def _partitionedBuild(self, ctxt, pipe, prog, ins, est):
+        """Unique build without a global CAS per key, two launches of the
+        build pipeline plus a spill step.
+
+        Device-scope CAS is what bounds a hash build on MI355X (c1's dim
+        build: 300 k inserts, 25 us; with plain stores instead, 9 us).
+        PHASE 0 runs the pipeline; a row's home slot decides its partition
+        (np contiguous slot ranges of `span` slots), and workgroup w of the
+        launch appends (tag, stored columns) records to its own segment of
+        that partition through an LDS cursor.  Rows that do not fit their
+        segment, and the tag-0 key, are inserted directly (CAS).  PHASE 1
+        gives workgroup b partition b: it copies the tags of its slot range
+        into LDS (slots taken by direct inserts stay taken), claims each
+        record's first free slot from its home slot in that LDS image, and
+        writes the tag and stored columns to the claimed slot with plain
+        stores -- no other workgroup writes that range.  A record whose probe
+        run leaves the range goes to a spill list that a last kernel inserts
+        with the ordinary CAS probe; every slot from its home slot to the
+        range end is taken, so it lands where a sequential build would put
+        it.  The probe side reads the same table layout either way."""
+        from dogqc_amd.hipTranslator import staticLdsBytes, WAVE
+        a = self.algExpr
+        J = a.opId
+        idx = self.ht["index"]
+        B, U = pipe.bs, pipe.U
+        tt = self.tagType
+        tb = 4 if self.tag32 else 8
+        words = 1 + len(self.stored)
+        fields = ins["fields"]
+        prog.hostVars["bpart%d" % idx] = (
+            "{ const int32_t req = t->part_build; const uint64_t cap = (uint64_t)L.htcap[%d];\n"
+            "  uint32_t np = 0u;\n"
+            "  if (req > 0) np = std::min<uint32_t>(1024u, std::max<uint32_t>(2u, pow2floor((uint64_t)req)));\n"
+            "  uint64_t span = np ? (cap + np - 1u) / np : 0u;\n"
+            "  while (np && np < 1024u && span * %du > %du) { np *= 2u; span = (cap + np - 1u) / np; }\n"
+            "  if (span * %du > %du) np = 0u;\n"
+            "  L.apart[%d] = np; L.bspan[%d] = np ? (uint32_t)span : 0u;\n"
+            "  if (np) {\n"
+            "    const uint32_t seg = std::min<uint32_t>(2048u, gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, 1));\n"
+            "    const double mu = (double)std::max<int64_t>(1, %s) / ((double)seg * np);\n"
+            "    L.apseg[%d] = seg;\n"
+            "    L.apcap[%d] = std::min<int64_t>(((int64_t)1 << 31) / seg, (int64_t)(mu * 1.25 + 4.0 * std::sqrt(mu)) + 16);\n"
+            "    L.bspill[%d] = (uint32_t)std::min<uint64_t>(1u << 30, std::max<uint64_t>(4096u, cap / 64u));\n"
+            "  } }"
+            % (idx, tb, self.PART_BUILD_SPAN_MAX,
+               tb, self.PART_BUILD_SPAN_MAX, idx, idx, pipe.pipeId, pipe.pipeId, pipe.defaultGrid,
+               pipe.params["n"][1], B * U, est, idx, idx, idx))
+        recs = "(uint64_t)L.apart[%d] * L.apseg[%d] * (uint64_t)L.apcap[%d] * %d" % (idx, idx, idx, words)
+        fnp = pipe.param("bnp%d" % J, "uint32_t", "L.apart[%d]" % idx)
+        fseg = pipe.param("bseg%d" % J, "uint32_t", "L.apseg[%d]" % idx)
+        fpcap = pipe.param("bpcap%d" % J, "uint32_t", "L.apcap[%d]" % idx)
+        fspan = pipe.param("bspan%d" % J, "uint32_t", "L.bspan[%d]" % idx)
+        bcnt = prog.buffer("bpc%d" % J, CType.UINT, "(uint64_t)L.apart[%d] * L.apseg[%d]" % (idx, idx))
+        fcnt = pipe.param("bpc%d" % J, "uint32_t*", "B_" + bcnt)
+        brec = prog.buffer("bprec%d" % J, CType.UINT64, recs)
+        frec = pipe.param("bprec%d" % J, "uint64_t*", "B_" + brec)
+        swords = 2 + len(self.stored)
+        bsp = prog.buffer("bsp%d" % J, CType.UINT64, "(uint64_t)L.bspill[%d] * %d" % (idx, swords))
+        bspc = prog.buffer("bspc%d" % J, CType.UINT, "1", "0x0ull")
+        fsp = pipe.param("bsp%d" % J, "uint64_t*", "B_" + bsp)
+        fspc = pipe.param("bspc%d" % J, "uint32_t*", "B_" + bspc)
+        fspcap = pipe.param("bspcap%d" % J, "uint32_t", "L.bspill[%d]" % idx)
+        pipe.annotations.append("PB%d" % J)
+        cur, pref, scan = "bcur%d" % J, "bpref%d" % J, "bscan%d" % J
+        from dogqc_amd.hipTranslator import AggregationTranslator as AT
+        pipe.staticLds.append("__shared__ uint32_t %s[%d];" % (cur, 1024))
+        pipe.staticLds.append("__shared__ uint32_t %s[%d];" % (pref, AT.PART_SEG_MAX + 1))
+        pipe.staticLds.append("__shared__ uint32_t %s[%d];" % (scan, 1024 // WAVE + 1))
+        img = "bimg%d" % J
+        pro = pipe.prologue
+        pro.emit("%s* %s = (%s*)dqc_dyn_lds;" % (tt, img, tt))
+        pro.emit("const uint32_t blo%d = std::min<uint32_t>(%s, blockIdx.x * %s);" % (J, ins["fcap"], fspan))
+        pro.emit("const uint32_t bhi%d = std::min<uint32_t>(%s, blo%d + %s);" % (J, ins["fcap"], J, fspan))
+        # (uniform branch: no barrier when the build is not partitioned)
+        pro.open("if (%s > 0u)" % fnp)
+        pro.emit("for (uint32_t i = threadIdx.x; i < %s; i += %du) %s[i] = 0u;" % (fnp, B, cur))
+        pro.open("if constexpr (PHASE == 1)")
+        pro.emit("for (uint32_t i = threadIdx.x; i < bhi%d - blo%d; i += %du) %s[i] = %s[blo%d + i];"
+                 % (J, J, B, img, ins["ftags"], J))
+        pro.close()
+        pro.emit("__syncthreads();")
+        pro.close()
+        jtag = ins["jtag"]
+        need = pipe.reg("bneed%d" % J, "int")
+        # PHASE 0: everything before this point, then the insert or the scatter
+        pipe.code.lines.insert(0, "    " * (pipe.code.depth - 1) + "if constexpr (PHASE == 0) {")
+        pipe.code.depth += 1
+        pipe.code.open("if (%s == 0u)" % fnp)
+        self._directInsert(pipe, ins, "active[u]")
+        pipe.code.close()
+        pipe.code.open("else")
+        pipe.emit("// append to this workgroup's segment of the row's partition")
+        values = [pipe.access(att) for att in self.stored]
+        with pipe.rowLoop():
+            pipe.emit("%s = 0;" % need)
+            pipe.code.open("if (active[u])")
+            self._hashCode(pipe, self.buildKeys, jtag, "jh%d" % J, tagIsReg=True)
+            if ins["bloom"] is not None:
+                fbf, fbfw = ins["bloom"]
+                pipe.emit("if (%s) dqc::bloom_set(%s, %s, jh%d);" % (fbfw, fbf, fbfw, J))
+            pipe.emit("const uint32_t pb = %s != 0u ? dqc::slot_of(jh%d, %s) / %s : 0xffffffffu;"
+                      % (jtag, J, ins["fcap"], fspan))
+            pipe.emit("const uint32_t pos = pb != 0xffffffffu ? atomicAdd(&%s[pb], 1u) : 0xffffffffu;" % cur)
+            pipe.code.open("if (pos < %s)" % fpcap)
+            pipe.emit("uint64_t* rec = %s + (((uint64_t)pb * %s + blockIdx.x) * %s + pos) * %du;"
+                      % (frec, fseg, fpcap, words))
+            pipe.emit("rec[0] = (uint64_t)%s;" % jtag)
+            for k, (att, v) in enumerate(zip(self.stored, values)):
+                pipe.emit("rec[%d] = dqc::to_word<%s>(%s);" % (k + 1, pipe.attrs[att.id].ctype, v))
+            pipe.code.close()
+            pipe.code.open("else")
+            pipe.emit("// segment full, or the tag-0 key: insert directly")
+            pipe.emit("%s = 1;" % need)
+            self._firstCas(pipe, ins)
+            pipe.code.close()
+            pipe.code.close()
+        self._resolveInsert(pipe, ins, "%s" % need, values)
+        pipe.code.close()   # end of the scatter
+        pipe.code.close()   # end of PHASE 0
+        # PHASE 1: the loop index is a record of this workgroup's partition
+        pipe.code.open("else")
+        rr = pipe.reg("bprec%d" % J, "uint64_t")
+        pipe.emit("dqc::part_records<%d>(%s, %s, blockIdx.x, %s, %s, active, %s);"
+                  % (U, pref, fseg, fpcap, pipe.tid, rr[:-3]))
+        pw = [pipe.reg("bpw%d_%d" % (J, k), "uint64_t") for k in range(len(self.stored))]
+        with pipe.rowLoop():
+            pipe.code.open("if (active[u])")
+            pipe.emit("const uint64_t* rec = %s + %s * %du;" % (frec, rr, words))
+            pipe.emit("%s = (%s)rec[0];" % (jtag, tt))
+            for k, r in enumerate(pw):
+                pipe.emit("%s = rec[%d];" % (r, k + 1))
+            pipe.code.close()
+        with pipe.rowLoop():
+            pipe.code.open("if (active[u])")
+            if self.tag32:
+                pipe.emit("const uint64_t jh = dqc::mix64(dqc::TAG_TOP | (uint64_t)(%s - 1u));" % jtag)
+            else:
+                pipe.emit("const uint64_t jh = dqc::mix64(%s);" % jtag)
+            pipe.emit("const uint32_t hs = dqc::slot_of(jh, %s);" % ins["fcap"])
+            pipe.emit("int64_t js = -1;")
+            pipe.emit("bool spill = false;")
+            pipe.code.open("for (uint32_t q = hs, n = 0;; ++q, ++n)")
+            pipe.emit("if (q >= bhi%d) { spill = true; break; }" % J)
+            pipe.emit("if (n >= dqc::MAX_PROBE) break;")
+            pipe.emit("if (dqc::lds_cas<%s>(&%s[q - blo%d], (%s)0, %s) == (%s)0) { js = (int64_t)q; break; }"
+                      % (tt, img, J, tt, jtag, tt))
+            pipe.code.close()
+            pipe.code.open("if (js >= 0)")
+            pipe.emit("%s[js] = %s;" % (ins["ftags"], jtag))
+            for f, att, r in zip(fields, self.stored, pw):
+                pipe.emit("%s[js] = dqc::from_word<%s>(%s);" % (f, pipe.attrs[att.id].ctype, r))
+            pipe.code.close()
+            pipe.code.open("else if (spill)")
+            pipe.emit("const uint32_t sp = atomicAdd(%s, 1u);" % fspc)
+            pipe.code.open("if (sp < %s)" % fspcap)
+            pipe.emit("uint64_t* srec = %s + (uint64_t)sp * %du;" % (fsp, swords))
+            pipe.emit("srec[0] = (uint64_t)%s; srec[1] = hs;" % jtag)
+            for k, r in enumerate(pw):
+                pipe.emit("srec[%d] = %s;" % (k + 2, r))
+            pipe.code.close()
+            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
+                      % (1 << idx))
+            pipe.code.close()
+            pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); atomicOr(&st->ht_overflow_mask, %du); }"
+                      % (1 << idx))
+            pipe.code.close()
+        pipe.code.close()
+        # PHASE 0 end: the segment counts
+        pipe.epilogue.open("if (PHASE == 0 && %s > 0u)" % fnp)
+        pipe.epilogue.emit("__syncthreads();")
+        pipe.epilogue.emit("for (uint32_t i = threadIdx.x; i < %s; i += %du) %s[(uint64_t)i * %s + blockIdx.x] = "
+                           "std::min<uint32_t>(%s[i], %s);" % (fnp, B, fcnt, fseg, cur, fpcap))
+        pipe.epilogue.close()
+        pipe.phased = True
+        pipe.phasedHt = self.ht
+        pipe.phaseBetween = []
+        pipe.partAgg = {"np": fnp, "cap": fpcap, "count": fcnt, "nseg": fseg, "pref": pref, "scan": scan,
+                        "hostNp": "L.apart[%d]" % idx, "hostNseg": "L.apseg[%d]" % idx}
+        pipe.dynLdsExpr = "(uint64_t)L.bspan[%d] * %du" % (idx, tb)
+        cols = [(b, ct) for b, ct in self.storedBuffers]
+
+        def spillStep(prog, J=J, idx=idx, tt=tt, swords=swords, cols=cols, bsp=bsp, bspc=bspc):
+            lines = ["if (L.apart[%d] > 0u) {" % idx, "dqc::spill_cols sc{};", "sc.n = %du;" % len(cols)]
+            for k, (b, ct) in enumerate(cols):
+                lines.append("sc.ptr[%d] = (void*)B_%s; sc.size[%d] = (uint32_t)sizeof(%s);" % (k, b, k, ct))
+            lines.append("hipLaunchKernelGGL(dqc::dqc_spill_insert<%s>, dim3(64), dim3(256), 0, s, (%s*)B_jht%d_tags, "
+                         "(uint32_t)L.htcap[%d], (const uint64_t*)B_%s, %du, (const uint32_t*)B_%s, "
+                         "(uint32_t)L.bspill[%d], sc, &st->flags, &st->ht_overflow_mask, %du);"
+                         % (tt, tt, J, idx, bsp, swords, bspc, idx, 1 << idx))
+            lines.append("}")
+            return lines
+        pipe.phaseAfter = [{"name": "dqc_spill_insert", "host": spillStep}]
 
     def _storeEntry(self, pipe, idx, fecap, fields, pos):
         pipe.code.open("if (%s < %s)" % (pos, fecap))

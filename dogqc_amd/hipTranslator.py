@@ -515,6 +515,8 @@ class Program(object):
         h.emit("    int64_t apcap[%d];" % max(1, len(self.hts)))
         h.emit("    uint32_t apseg[%d];" % max(1, len(self.hts)))
         h.emit("    uint32_t bfw[%d];" % max(1, len(self.hts)))
+        h.emit("    uint32_t bspan[%d];" % max(1, len(self.hts)))
+        h.emit("    uint32_t bspill[%d];" % max(1, len(self.hts)))
         h.emit("    uint32_t bfp[%d];" % max(1, len(self.pipes)))
         h.emit("    int64_t apgroups[%d];" % max(1, len(self.hts)))
         h.emit("};")
@@ -561,7 +563,7 @@ class Program(object):
             h.emit("(void)est_%d;" % opId)
         h.emit("for (int i = 0; i < %d; ++i) L.bfp[i] = 0u;" % max(1, len(self.pipes)))
         h.emit("for (int i = 0; i < %d; ++i) { L.arep[i] = 1u; L.apart[i] = 0u; L.apbits[i] = 0u; L.apcap[i] = 0; "
-               "L.apseg[i] = 0u; L.apgroups[i] = 0; L.bfw[i] = 0u; }"
+               "L.apseg[i] = 0u; L.apgroups[i] = 0; L.bfw[i] = 0u; L.bspan[i] = 0u; L.bspill[i] = 0u; }"
                % max(1, len(self.hts)))
         for name, stmt in self.hostVars.items():
             # a callable is rendered now, when every pipeline is complete
@@ -722,7 +724,8 @@ def pipelineLaunch(prog, pipe, phase=None, rowsExpr=None, perCu=8):
     if pa is not None and phase == 0:
         # rows are scattered to partitions: no LDS group table in this pass
         lines.append("const bool part = %s > 0u;" % pa["hostNp"])
-        lines.append("if (part) kp.lw = 0u;")
+        if "lw" in pipe.params:
+            lines.append("if (part) kp.lw = 0u;")
         keep = "(size_t)0" if pipe.ldsFilter is None else "(size_t)(%s)" % pipe.ldsFilter["host"]
         lds = "(part ? %s : %s)" % (keep, lds)
     lines.append("const unsigned g = gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, %s);"
@@ -1065,6 +1068,8 @@ def closePipeline(ctxt, pipe):
         rows1 = pipe.phaseRows[1][0] if 1 in pipe.phaseRows else None
         prog.steps.append({"name": pipe.name + "<1>",
                            "host": lambda p, pipe=pipe, r=rows1: pipelineLaunch(p, pipe, 1, rowsExpr=r)})
+        for s in getattr(pipe, "phaseAfter", None) or []:
+            prog.steps.append(s)
     else:
         prog.steps.append({"name": pipe.name, "host": lambda p, pipe=pipe: pipelineLaunch(p, pipe)})
     ctxt.pipe = None
@@ -1721,8 +1726,11 @@ class AggregationTranslator(UnaryTranslator):
         pipe.staticLds.append("__shared__ uint32_t %s[%d];" % (cur, self.PART_MAX))
         pipe.staticLds.append("__shared__ uint32_t %s[%d];" % (pref, self.PART_SEG_MAX + 1))
         pipe.staticLds.append("__shared__ uint32_t %s[%d];" % (scan, 1024 // WAVE + 1))
+        # (uniform branch: no barrier when the group-by is not partitioned)
+        pipe.prologue.open("if (%s > 0u)" % fnp)
         pipe.prologue.emit("for (uint32_t i = threadIdx.x; i < %s; i += %du) %s[i] = 0u;" % (fnp, B, cur))
         pipe.prologue.emit("__syncthreads();")
+        pipe.prologue.close()
         # PHASE 0: everything before this point, then the update or the scatter
         pipe.code.lines.insert(0, "    " * (pipe.code.depth - 1) + "if constexpr (PHASE == 0) {")
         pipe.code.depth += 1
@@ -1775,7 +1783,7 @@ class AggregationTranslator(UnaryTranslator):
         self._exactUpdate(pipe, ptag, vals1, upd, "(gh << %s)" % fbits)
         pipe.code.close()
         # PHASE 0 end: the segment counts
-        pipe.epilogue.open("if constexpr (PHASE == 0)")
+        pipe.epilogue.open("if (PHASE == 0 && %s > 0u)" % fnp)
         pipe.epilogue.emit("__syncthreads();")
         pipe.epilogue.emit("for (uint32_t i = threadIdx.x; i < %s; i += %du) %s[(uint64_t)i * %s + blockIdx.x] = "
                            "std::min<uint32_t>(%s[i], %s);" % (fnp, B, fcnt, fseg, cur, fpcap))

@@ -23,10 +23,11 @@ from dogqc_amd.kernel import KernelCall
 
 
 @contextlib.contextmanager
-def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None):
+def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBuild=False):
     old = (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
-           KernelCall.defaultRowsPerThread)
+           KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild)
     EquiJoinTranslator.usePushDownJoin = pushdown
+    EquiJoinTranslator.usePartitionedBuild = partBuild
     if block is not None:
         KernelCall.defaultBlockSize = block
     if threshold is not None:
@@ -37,7 +38,7 @@ def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None):
         yield
     finally:
         (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
-         KernelCall.defaultRowsPerThread) = old
+         KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild) = old
 
 
 # ------------------------------------------------------------- feature plans
@@ -136,6 +137,17 @@ def _featurePlan(name, alg):
                                              alg.selection(scal.EqualsExpr(_A("t_cat"), _C("b", Type.CHAR)),
                                                            alg.scan("t")))))
         return p, {4: {"multimatch": False}, 6: {"numgroups": 16}}
+    if name == "f_join_unique_lowest":
+        # a PK join with the build side's selectivity set far too low: the
+        # tables are sized for ~1 build row, so every structure that is sized
+        # from it (a partitioned build's segments) overflows
+        p = alg.aggregation(["u_grp"], [(Reduction.SUM, "w", "sw"), (Reduction.COUNT, "", "c")],
+                            alg.map("w", scal.MulExpr(_A("u_w"), _A("t_val")),
+                                    alg.join(("u_id", "t_id"),
+                                             alg.selection(scal.LargerExpr(_A("u_grp"), _C("-1000", Type.INT)),
+                                                           alg.scan("u")),
+                                             alg.scan("t"))))
+        return p, {2: {"selectivity": 0.001}, 4: {"multimatch": False}, 6: {"numgroups": 128}}
     if name == "f_groupjoin":
         # group-by on the key of a unique join: accumulates at the join slot
         p = alg.aggregation(["u_id"], [(Reduction.SUM, "t_val", "s"), (Reduction.MIN, "t_val", "mn"),
@@ -266,6 +278,7 @@ def _featurePlan(name, alg):
 
 
 FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
+                 "f_join_unique_lowest",
                  "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
                  "f_groupby_int64", "f_outer", "f_outer_project", "f_join_neg", "f_anti_neg",
                  "f_cross", "f_cross_pairs", "f_like_patterns"]
@@ -291,6 +304,17 @@ PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_pr
 # TPC-H Q1-Q22 (dogqc_amd/tpch.py)
 for _q in range(1, 23):
     PLANS["tpch%d" % _q] = ("tpch", {"query": _q})
+
+# plans translated with the partitioned unique build (tuning part_build)
+PB_FEATURES = ["f_groupjoin", "f_join_unique_lowest", "f_semi", "f_temp", "f_join_neg", "f_anti_neg"]
+PB_TPCH = [3, 5, 9, 18, 21]
+for _n in PB_FEATURES:
+    PLANS[_n + "_pb"] = ("feature", {"part_build": True, "base": _n})
+for _q in PB_TPCH:
+    PLANS["tpch%d_pb" % _q] = ("tpch", {"query": _q, "part_build": True})
+PLANS["c1_pb"] = ("c1", {"part_build": True})
+PLANS["c1_refill_pb"] = ("c1", {"buffers": [4], "part_build": True})
+PLANS["c3_refill_pb"] = ("c3", {"buffers": [6], "part_build": True})
 # batched serving plans (dogqc_amd/batching.py): one run for several queries
 for _c in ("c0", "c1", "c2", "c3"):
     PLANS["%s_batch" % _c] = ("batch", {"config": _c})
@@ -346,7 +370,8 @@ def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=
     alg, p, cfg, opts = build(name, acc)
     plan = alg.resolveAlgebraPlan(p, cfg)
     with knobs(pushdown=opts.get("pushdown", False), block=block or opts.get("block"),
-               threshold=opts.get("threshold"), rowsPerThread=rowsPerThread):
+               threshold=opts.get("threshold"), rowsPerThread=rowsPerThread,
+               partBuild=opts.get("part_build", False)):
         comp = HipCompiler(alg, "gfx950", decimalRepr)
         comp.setBuffers(opts.get("buffers", []))
         comp.setProfilers(opts.get("profilers", []))

@@ -45,7 +45,7 @@ class Tuning(C.Structure):
                 ("temp_capacity", C.c_int64 * MAX_TEMPS), ("split_capacity", C.c_int64 * MAX_HTS),
                 ("split_threshold", C.c_int32 * MAX_PIPES), ("profile", C.c_int32), ("xcd_replicas", C.c_int32),
                 ("census", C.c_int32), ("part_agg", C.c_int32), ("join_filter", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("part_build", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Status(C.Structure):

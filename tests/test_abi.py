@@ -81,6 +81,7 @@ def test_struct_sizes_match_header():
         == 1640
     assert rt.Tuning.part_agg.offset == rt.Tuning.census.offset + 4
     assert rt.Tuning.join_filter.offset == rt.Tuning.part_agg.offset + 4
+    assert rt.Tuning.part_build.offset == rt.Tuning.join_filter.offset + 4
     assert ctypes.sizeof(rt.Status) == 16 + 8 + 8 * 32 + 8 * 8 + 8 * 32 + 8 * 32 == 856
     assert ctypes.sizeof(rt.Layout) == 8 + 4 + 4 + 8 * 32 * 2 + 8 + 4 + 4 + 8 * 64 + 8 + 4 + 4 + 8 * 8 + 8 * 64 \
         + 8 * 32 == 1904

@@ -534,3 +534,55 @@ def test_join_filter_c3(kb):
 @pytest.mark.parametrize("kb", [-1, 1, 64])
 def test_join_filter_tpch10(kb):
     check("tpch10", tpchAcc(0.05), tuning=filterTuning(kb), storeKey=("tpch", 0.05), label="kb=%d" % kb)
+
+
+# ------------------------------------------------------------ partitioned build
+
+PB_FEATURES = planlib.PB_FEATURES
+PB_TPCH = planlib.PB_TPCH
+
+
+def buildTuning(parts):
+    t = rt.newTuning()
+    t.part_build = parts
+    return t
+
+
+@pytest.mark.parametrize("parts", [-1, 2, 64, 1024])
+@pytest.mark.parametrize("name", PB_FEATURES)
+def test_partitioned_build_features(name, parts):
+    """Unique builds forced through the partitioned build (tuning part_build
+    = n partitions): records scattered to slot-range partitions, each range
+    built in LDS and written with plain stores.  With 1024 partitions a small
+    table's ranges are a slot or two wide, so most keys spill into the CAS
+    pass; f_join_unique_lowest sizes the segments for ~1 build row, so most
+    rows overflow them and are inserted directly."""
+    name += "_pb"
+    ann = [a for p in module(name).desc["pipelines"] for a in p["annotations"]]
+    assert any(a.startswith("PB") for a in ann), name
+    for size in [(5000, 700), (200, 50), (1, 1), (0, 0)]:
+        check(name, planlib.featureData(*size), tuning=buildTuning(parts),
+              label="parts=%d t=%d u=%d" % ((parts,) + size))
+
+
+@pytest.mark.parametrize("parts", [2, 1024])
+@pytest.mark.parametrize("q", PB_TPCH)
+def test_partitioned_build_tpch(q, parts):
+    check("tpch%d_pb" % q, tpchAcc(0.05), tuning=buildTuning(parts), storeKey=("tpch", 0.05), label="parts=%d" % parts)
+
+
+@pytest.mark.parametrize("parts", [0, 16, 1024])
+@pytest.mark.parametrize("plan", ["c1_pb", "c1_refill_pb"])
+def test_partitioned_build_c1(plan, parts):
+    for rows in [{"dim": 300000, "fact": 400000}, {"dim": 1000, "fact": 5000}]:
+        check(plan, workloadAcc("c1", rows, seed=61), tuning=buildTuning(parts), label="parts=%d %r" % (parts, rows))
+
+
+@pytest.mark.parametrize("parts", [0, 64])
+def test_partitioned_build_c3(parts):
+    print("gpu case: c3_refill part_build=%d" % parts, flush=True)
+    acc = workloadAcc("c3", {"sales": 200000, "item": 200000, "cust": 100000, "nation": 25}, seed=62)
+    got, ny, _ = c3Run(acc, tuning=buildTuning(parts), plan="c3_refill_pb")
+    want, wantNy = c3Oracle(acc)
+    assertSameResult(got, want, {"revenue"}, label="c3 by item parts=%d" % parts)
+    assertSameResult(ny, wantNy, {"revenue"}, label="c3 by nation, year parts=%d" % parts)
