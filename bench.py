@@ -115,8 +115,7 @@ def cpuBaseline(workload, g, seconds):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path; use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c4":
         return benchServing(args)

@@ -114,3 +114,7 @@ def buildAll(verbose=True):
                                                                                  time.time() - t0),
               file=sys.stderr)
     return paths
+
+
+if __name__ == "__main__":
+    buildAll()

@@ -36,7 +36,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(GEN_SO):
-            raise RuntimeError("libdqcgen.so is not built (run __graft_entry__.build())")
+            raise RuntimeError("libdqcgen.so is not built (run python3 -m dogqc_amd.build)")
         rt.lib()
         L = C.CDLL(GEN_SO)
         vp, i64, u64, u32, i32 = C.c_void_p, C.c_int64, C.c_uint64, C.c_uint32, C.c_int32

@@ -154,7 +154,7 @@ def importLib():
     global _importLib
     if _importLib is None:
         if not os.path.exists(IMPORT_SO):
-            raise RuntimeError("libdqcimport.so is not built (run __graft_entry__.build())")
+            raise RuntimeError("libdqcimport.so is not built (run python3 -m dogqc_amd.build)")
         L = C.CDLL(IMPORT_SO)
         L.dqc_tbl_rows.argtypes = [C.c_char_p, C.POINTER(C.c_int64)]
         L.dqc_import_tbl.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_char_p), C.c_int32,

@@ -102,7 +102,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIBPATH):
-            raise DqcError("libdqcrt.so is not built (run __graft_entry__.build())")
+            raise DqcError("libdqcrt.so is not built (run python3 -m dogqc_amd.build)")
         L = C.CDLL(LIBPATH)
         L.dqc_last_error.restype = C.c_char_p
         L.dqc_module_describe.restype = C.c_char_p
