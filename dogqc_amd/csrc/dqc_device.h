@@ -837,8 +837,10 @@ __global__ void __launch_bounds__(SCAN_BS) dqc_mm_scan_blocks(uint64_t* block_su
     if (threadIdx.x == 0) *total_out = carry;
 }
 
+// With keep_counts the counts stay (an insert pass that places entries by
+// precomputed ranks needs no cursors); otherwise they are cleared.
 __global__ void __launch_bounds__(SCAN_BS) dqc_mm_scan_apply(uint32_t* dir, uint64_t n,
-                                                              const uint64_t* block_sums) {
+                                                              const uint64_t* block_sums, uint32_t keep_counts) {
     __shared__ uint64_t sh[SCAN_BS / WAVE + 1];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
@@ -856,7 +858,7 @@ __global__ void __launch_bounds__(SCAN_BS) dqc_mm_scan_apply(uint32_t* dir, uint
         uint64_t ix = base + i;
         if (ix < n) {
             dir[2 * ix] = (uint32_t)run;
-            dir[2 * ix + 1] = 0u;
+            if (!keep_counts) dir[2 * ix + 1] = 0u;
         }
         run += v[i];
     }

@@ -209,6 +209,7 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.phasedHt = ht
         jb = pipe.reg("jb%d" % J, "uint32_t")
         if pipe.loopOps == 0 and pipe.innerDepth == 0:
+            self._prog = prog
             self._multiBuildStaged(pipe, J, idx, fdir, fcap, fecap, fields, jb)
             return
         with pipe.rowLoop():
@@ -494,68 +495,73 @@ class EquiJoinTranslator(BinaryTranslator):
                   "atomicOr(&st->mm_overflow_mask, %du); }" % (1 << idx))
 
     def _multiBuildStaged(self, pipe, J, idx, fdir, fcap, fecap, fields, jb):
-        """Count and insert with per-trip workgroup aggregation: rows of one
-        loop trip that fall into the same bucket share one LDS counter, so a
-        bucket costs one global atomic per workgroup and trip (the insert pass
-        reserves a range and hands out ranks from LDS) instead of one per
-        row; skewed keys otherwise serialise on a single address in L2."""
+        """Count pass that also ranks every row within its bucket, insert
+        pass without atomics.
+
+        PHASE 0: rows of one loop trip that fall into the same bucket share
+        one LDS counter, so a bucket costs one global atomic per workgroup and
+        trip instead of one per row (skewed keys otherwise serialise on one
+        address); that atomic returns the group's first rank in the bucket,
+        and each row keeps base + its rank in the group (rows without an LDS
+        slot take their rank from their own atomic).  The bucket scan keeps
+        the counts.  PHASE 1 re-runs the pipeline -- the same rows at the
+        same indices -- and stores each entry at bucket offset + rank: no
+        atomics and no barriers (the ranked insert replaced one returning
+        global atomic per bucket and trip)."""
         W = 1
         while W < pipe.bs * pipe.U:
             W *= 2
+        prog = self._prog
+        self.ht["ranked"] = True
+        rb = prog.buffer("mmr%d" % J, CType.UINT, pipe.params["n"][1])
+        frank = pipe.param("mmr%d" % J, "uint32_t*", "B_" + rb)
         pipe.staticLds.append("__shared__ uint32_t mmk%d[%d];" % (J, W))
         pipe.staticLds.append("__shared__ uint32_t mmc%d[%d];" % (J, W))
         pipe.staticLds.append("__shared__ uint32_t mmb%d[%d];" % (J, W))
+        pipe.prologue.open("if constexpr (PHASE == 0)")
         pipe.prologue.emit("for (uint32_t i = threadIdx.x; i < %du; i += %du) { mmk%d[i] = dqc::LKEY_EMPTY; "
                            "mmc%d[i] = 0u; }" % (W, pipe.bs, J, J))
         pipe.prologue.emit("__syncthreads();")
+        pipe.prologue.close()
         ls = pipe.reg("mmls%d" % J, "int")
         rank = pipe.reg("mmrank%d" % J, "uint32_t")
-        # pass A: bucket, LDS slot and rank within the trip (or a global atomic)
+        tid = pipe.tid + "[u]"
         with pipe.rowLoop():
-            pipe.emit("%s = -1;" % ls)
             pipe.code.open("if (active[u])")
             self._hashCode(pipe, self.buildKeys, "jtag%d" % J, "jh%d" % J)
             pipe.emit("%s = dqc::slot_of(jh%d, %s);" % (jb, J, fcap))
+            pipe.code.close()
+        pipe.code.open("if constexpr (PHASE == 0)")
+        # bucket, LDS slot and rank within the trip (or the row's own atomic)
+        with pipe.rowLoop():
+            pipe.emit("%s = -1;" % ls)
+            pipe.code.open("if (active[u])")
             pipe.emit("%s = dqc::lds_claim32(mmk%d, %du, %s, 8);" % (ls, J, W - 1, jb))
-            pipe.code.open("if (%s >= 0)" % ls)
-            pipe.emit("%s = atomicAdd(&mmc%d[%s], 1u);" % (rank, J, ls))
+            pipe.emit("%s = %s >= 0 ? atomicAdd(&mmc%d[%s], 1u) : atomicAdd(&%s[2u * %s + 1u], 1u);"
+                      % (rank, ls, J, ls, fdir, jb))
             pipe.code.close()
-            pipe.code.open("else")
-            pipe.code.open("if constexpr (PHASE == 0)")
-            pipe.emit("atomicAdd(&%s[2u * %s + 1u], 1u);" % (fdir, jb))
-            pipe.code.close()
-            pipe.code.open("else")
-            pipe.emit("const uint32_t jpos = %s[2u * %s] + atomicAdd(&%s[2u * %s + 1u], 1u);" % (fdir, jb, fdir, jb))
-            self._storeEntry(pipe, idx, fecap, fields, "jpos")
-            pipe.code.close()
-            pipe.code.close()
-            pipe.code.close()
-        # workgroup: one global atomic per distinct bucket of the trip
+        # one global atomic per distinct bucket of the trip: the group's base rank
         pipe.emit("__syncthreads();")
         pipe.code.open("for (uint32_t i = threadIdx.x; i < %du; i += %du)" % (W, pipe.bs))
         pipe.emit("const uint32_t k = mmk%d[i];" % J)
-        pipe.code.open("if (k != dqc::LKEY_EMPTY)")
-        pipe.code.open("if constexpr (PHASE == 0)")
-        pipe.emit("atomicAdd(&%s[2u * k + 1u], mmc%d[i]);" % (fdir, J))
-        pipe.code.close()
-        pipe.code.open("else")
-        pipe.emit("mmb%d[i] = %s[2u * k] + atomicAdd(&%s[2u * k + 1u], mmc%d[i]);" % (J, fdir, fdir, J))
-        pipe.code.close()
-        pipe.code.close()
+        pipe.emit("if (k != dqc::LKEY_EMPTY) mmb%d[i] = atomicAdd(&%s[2u * k + 1u], mmc%d[i]);" % (J, fdir, J))
         pipe.code.close()
         pipe.emit("__syncthreads();")
-        # pass B (insert): rows placed through LDS take their reserved position
         with pipe.rowLoop():
-            pipe.code.open("if constexpr (PHASE == 1)")
-            pipe.code.open("if (active[u] && %s >= 0)" % ls)
-            pipe.emit("const uint32_t jpos = mmb%d[%s] + %s;" % (J, ls, rank))
-            self._storeEntry(pipe, idx, fecap, fields, "jpos")
-            pipe.code.close()
-            pipe.code.close()
+            pipe.emit("if (active[u]) %s[%s] = %s >= 0 ? mmb%d[%s] + %s : %s;" % (frank, tid, ls, J, ls, rank, rank))
         pipe.emit("__syncthreads();")
         pipe.emit("for (uint32_t i = threadIdx.x; i < %du; i += %du) { mmk%d[i] = dqc::LKEY_EMPTY; mmc%d[i] = 0u; }"
                   % (W, pipe.bs, J, J))
         pipe.emit("__syncthreads();")
+        pipe.code.close()
+        # PHASE 1: each entry at its bucket's offset plus its rank
+        pipe.code.open("else")
+        with pipe.rowLoop():
+            pipe.code.open("if (active[u])")
+            pipe.emit("const uint32_t jpos = %s[2u * %s] + %s[%s];" % (fdir, jb, frank, tid))
+            self._storeEntry(pipe, idx, fecap, fields, "jpos")
+            pipe.code.close()
+        pipe.code.close()
 
     # ----------------------------------------------------------- probe
     def _storedFields(self, pipe):

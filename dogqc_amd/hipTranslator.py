@@ -1095,7 +1095,7 @@ def mmScanSteps(ht):
         return ["const uint64_t nb = (uint64_t)L.htcap[%d];" % idx,
                 "const unsigned g = (unsigned)std::max<uint64_t>(1, (nb + dqc::SCAN_TILE - 1) / dqc::SCAN_TILE);",
                 "hipLaunchKernelGGL(dqc::dqc_mm_scan_apply, dim3(g), dim3(dqc::SCAN_BS), 0, s, "
-                "(uint32_t*)B_%s, nb, (const uint64_t*)B_%s);" % (dirb, sums)]
+                "(uint32_t*)B_%s, nb, (const uint64_t*)B_%s, %du);" % (dirb, sums, 1 if ht.get("ranked") else 0)]
 
     return [{"name": "dqc_mm_scan_reduce", "host": reduce_}, {"name": "dqc_mm_scan_blocks", "host": blocks},
             {"name": "dqc_mm_scan_apply", "host": apply_}]
