@@ -97,8 +97,11 @@ def plan(config, acc):
         # (nation, year) groups: 175 per member, on the LDS tier (4096 slots);
         # (segment, item) groups are bounded by the star rows (no hint: sized
         # from the temp table's row estimate, so large batches do not retry)
+        # and number ~106k per member at 1x: a partitioned group-by of 1024
+        # partitions instead of one global atomic per star row
         cfg = {j1.opId: {"multimatch": False, "selectivity": 0.06}, j2.opId: {"multimatch": False},
-               j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 176 * MEMBERS}}
+               j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 176 * MEMBERS},
+               byItem.opId: {"partitions": 1024}}
         return alg, [star, ny, byItem], cfg, {"buffers": [j1.opId]}
     raise KeyError(config)
 

@@ -1659,6 +1659,8 @@ class AggregationTranslator(UnaryTranslator):
         if any(acc["null"] for acc in self.accs):
             return False
         a = self.algExpr
+        if getattr(a, "partitions", None):
+            return True
         if getattr(a, "numgroupsFromCfg", False) and a.numgroups <= self.PART_MIN_GROUPS:
             return False
         return True
@@ -1685,11 +1687,15 @@ class AggregationTranslator(UnaryTranslator):
         idx = self.ht["index"]
         B, U = pipe.bs, pipe.U
         groupsKnown = "(int64_t)%d" % int(a.numgroups) if getattr(a, "numgroupsFromCfg", False) else "(int64_t)0"
+        cfgParts = 0
+        if getattr(a, "partitions", None):
+            cfgParts = min(self.PART_MAX, max(2, _pow2floor(int(a.partitions))))
         prog.hostVars["apart%d" % idx] = (
             "{ const int32_t req = t->part_agg;\n"
             "  const int64_t known = t->ht_capacity[%d] > 0 ? t->ht_capacity[%d] / 2 : %s;\n"
             "  uint32_t np = 0u;\n"
             "  if (req > 0) np = std::min<uint32_t>(%du, std::max<uint32_t>(2u, pow2floor((uint64_t)req)));\n"
+            "  else if (req == 0 && %d > 0) np = %du;\n"
             "  else if (req == 0 && known > %d) np = std::min<uint32_t>(%du, std::max<uint32_t>(2u, pow2ceil((uint64_t)(known / %d))));\n"
             "  uint32_t bits = 0u; while ((1u << bits) < np) ++bits;\n"
             "  L.apart[%d] = np; L.apbits[%d] = bits;\n"
@@ -1700,7 +1706,8 @@ class AggregationTranslator(UnaryTranslator):
             "    L.apseg[%d] = seg;\n"
             "    L.apcap[%d] = std::min<int64_t>(((int64_t)1 << 31) / seg, (int64_t)(mu * 1.25 + 4.0 * std::sqrt(mu)) + 16);\n"
             "  } }"
-            % (idx, idx, groupsKnown, self.PART_MAX, self.PART_MIN_GROUPS, self.PART_MAX, self.PART_GROUPS,
+            % (idx, idx, groupsKnown, self.PART_MAX, cfgParts, cfgParts, self.PART_MIN_GROUPS, self.PART_MAX,
+               self.PART_GROUPS,
                idx, idx, idx, est,
                self.PART_SEG_MAX, pipe.pipeId, pipe.pipeId, pipe.defaultGrid, pipe.params["n"][1], B * U,
                prog.estimate(a.child), idx, idx))

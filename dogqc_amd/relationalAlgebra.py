@@ -481,6 +481,9 @@ class Aggregation(UnaryAlgExpr):
         self.doGroup = len(self.groupAttributes) > 0
         self.numgroups = _cfgValue(cfg, self.opId, "numgroups", None)
         self.numgroupsFromCfg = self.numgroups is not None
+        # partitions of a partitioned group-by when the tuning leaves it to
+        # the plan (an extension: the reference has no such key)
+        self.partitions = _cfgValue(cfg, self.opId, "partitions", None)
         if self.numgroups is None:
             self.numgroups = childTupleNum
         if not self.doGroup:
