@@ -614,6 +614,7 @@ class Program(object):
         h.open('extern "C" int dqc_plan_enqueue(const dqc_plan_inputs* in, const dqc_plan_tuning* t, void* ws, uint64_t ws_bytes, void* stream, void* const* ev)')
         h.emit("Layout L; int rc = computeLayout(in, t, L); if (rc) return rc;")
         h.emit("if (!ws || ws_bytes < L.bytes) return DQC_ENOMEM;")
+        h.emit("if (((uintptr_t)ws & 15u) != 0u) return DQC_EINVAL;   // 16-byte loads and stores of table regions")
         h.emit("hipStream_t s = (hipStream_t)stream;")
         h.emit("unsigned char* W = (unsigned char*)ws;")
         h.emit("dqc_plan_status* st = (dqc_plan_status*)W;")
