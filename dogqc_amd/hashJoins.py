@@ -607,7 +607,7 @@ class EquiJoinTranslator(BinaryTranslator):
             for att in condAttrs:
                 st = pipe.attrs.get(att.id)
                 if att.id in buildIds:
-                    pipe.emit("%s[u] = %s;" % (st.reg, st.loader))
+                    pipe.emit("%s[u] = %s;" % (st.reg, st.loader() if callable(st.loader) else st.loader))
                     st.loaded = True
             parts.append("(%s)" % ctxt.exprEmitter().emit(a.conditions))
         return " && ".join(parts) if parts else "true"

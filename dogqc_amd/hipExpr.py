@@ -114,6 +114,9 @@ class ExprEmitter(object):
         ci = prog.column("__batch", e.table, None, "int64_t")
         f = pipe.param("c%d" % ci, "const int64_t*", "C_%d" % ci)
         nb = pipe.param("bmembers", "int64_t", "L.rows[%d]" % bt)
+        if pipe.loopOps:
+            # rows were reassigned to lanes: no common tile, one search per row
+            return "dqc::segment_of(%s, %s, %s[u])" % (f, nb, pipe.tid)
         return "dqc::segment_of(%s, %s, %s[u], base, %d)" % (f, nb, pipe.tid, pipe.bs * pipe.U)
 
     def _MaxExpr(self, e):

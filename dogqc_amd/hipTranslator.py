@@ -231,7 +231,9 @@ class Pipeline(object):
         with self.rowLoop():
             self.code.open("if (active[u])")
             for st in pending:
-                self.emit("%s[u] = %s;" % (st.reg, st.loader))
+                # a callable loader is rendered now, for the lanes as they are here
+                loader = st.loader() if callable(st.loader) else st.loader
+                self.emit("%s[u] = %s;" % (st.reg, loader))
                 st.loaded = True
             self.code.close()
 
@@ -1127,6 +1129,15 @@ class MapTranslator(UnaryTranslator):
         pipe = ctxt.pipe
         prog = ctxt.program
         pipe.emit("// map (opId %d)" % a.opId)
+        import dogqc_amd.scalarAlgebra as scal
+        if isinstance(a.expression, scal.SegmentExpr):
+            # a batch segment depends on the row index only: computed where it
+            # is first read, so rows filtered out before never search for it
+            emitter = ctxt.exprEmitter()
+            emitter.emit(a.expression)          # checks the scan and binds the prefix column
+            pipe.setLazy(a.mapAttr, lambda e=a.expression: emitter.emit(e), [pipe.tid])
+            self.parent.consume(ctxt)
+            return
         used = []
         _collectAttrs(a.expression, used)
         pipe.need(used)
