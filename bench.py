@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--max-heavy", type=int, default=1, help="c4: heavy queries running at once (0 = no limit)")
     ap.add_argument("--max-batch", type=int, default=16,
                     help="c4: waiting queries of one config run as one batched plan, up to this many (<= 1: off)")
-    ap.add_argument("--batch-ratio", type=int, default=4,
+    ap.add_argument("--batch-ratio", type=int, default=8,
                     help="c4: queries share a batch only if their driving rows are within this factor (0 = any)")
     ap.add_argument("--heavy-mb", type=float, default=64.0, help="c4: workspace size of a heavy query")
     ap.add_argument("--rows-scale", type=float, default=1.0)

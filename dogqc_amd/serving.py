@@ -367,7 +367,7 @@ def matchesOracle(config, got, want):
 
 
 def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None,
-              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=4):
+              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=8):
     """Serving benchmark on resident tables at `scale` x the base sizes.
     Returns a dict of capacity (serial / batched QPS), latency at a fixed
     Poisson rate (load x the serial capacity) and verification results."""
