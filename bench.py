@@ -72,6 +72,9 @@ def parse():
                     help="c4: waiting queries of one config run as one batched plan, up to this many (<= 1: off)")
     ap.add_argument("--batch-ratio", type=int, default=8,
                     help="c4: queries share a batch only if their driving rows are within this factor (0 = any)")
+    ap.add_argument("--batch-min-share", type=float, default=2.0,
+                    help="c4: a batch's largest query runs alone when the other members' driving rows sum to "
+                         "less than this share of its own (0 = never)")
     ap.add_argument("--heavy-mb", type=float, default=64.0, help="c4: workspace size of a heavy query")
     ap.add_argument("--rows-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -237,7 +240,7 @@ def benchServing(args):
     log = lambda m: print("[c4] " + m, file=sys.stderr, flush=True)
     r = serving.benchmark(scale=args.scale, nqueries=nq, slots=args.slots, arena_gb=args.arena_gb, log=log,
                           max_heavy=args.max_heavy, heavy_mb=args.heavy_mb, max_batch=args.max_batch,
-                          batch_ratio=args.batch_ratio)
+                          batch_ratio=args.batch_ratio, batch_min_share=args.batch_min_share)
     out = {
         "metric": METRIC,
         "value": r["batched"]["qps"],
@@ -254,7 +257,7 @@ def benchServing(args):
         "config": {"workload": SERVING["desc"], "resident_gb": r["resident_gb"], "scale": r["scale"],
                    "queries": nq, "slots": r["slots"], "arena_gb": r["arena_gb"], "max_heavy": r["max_heavy"],
                    "heavy_mb": r["heavy_mb"], "max_batch": r["max_batch"], "batch_ratio": r["batch_ratio"],
-                   "plans": r["plans"],
+                   "batch_min_share": r["batch_min_share"], "plans": r["plans"],
                    "parallelism": "single GPU"},
         "roofline": None,
         "serial_qps": r["serial"]["qps"],

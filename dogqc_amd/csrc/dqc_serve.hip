@@ -34,7 +34,7 @@
 #include "dogqc_amd.h"
 #include "dqc_runtime_internal.h"
 
-static_assert(sizeof(dqc_serve_config) == 48, "dqc_serve_config layout");
+static_assert(sizeof(dqc_serve_config) == 56, "dqc_serve_config layout");
 static_assert(sizeof(dqc_serve_request) == 96, "dqc_serve_request layout");
 static_assert(sizeof(dqc_serve_record) == 64 + sizeof(dqc_plan_layout) + sizeof(dqc_plan_status),
               "dqc_serve_record layout");
@@ -450,6 +450,20 @@ struct Server {
                     if (o.rec.arrival_ns > ta) break;
                     if (!o.internal && o.module == r.module && o.batchModule == r.batchModule && similar(o))
                         group.push_back(*jt);
+                }
+                // the largest member leaves a batch whose other members hold
+                // too little work to pay for the batch's extra cost
+                while (cfg.batch_min_share > 0 && group.size() >= 2) {
+                    size_t big = 0;
+                    double sum = 0;
+                    for (size_t k = 0; k < group.size(); ++k) {
+                        const double x = (double)drivingRows(*reqs[group[k]]);
+                        sum += x;
+                        if (x > (double)drivingRows(*reqs[group[big]])) big = k;
+                    }
+                    const double largest = (double)drivingRows(*reqs[group[big]]);
+                    if (sum - largest >= cfg.batch_min_share * largest) break;
+                    group.erase(group.begin() + (std::ptrdiff_t)big);
                 }
                 const int64_t bid = group.size() >= 2 ? makeBatch(group) : -1;
                 if (bid >= 0) {

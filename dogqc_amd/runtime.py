@@ -72,7 +72,8 @@ class QueryResult(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_retries", C.c_int32), ("arena_bytes", C.c_uint64),
                 ("bypass_ms", C.c_int32), ("size_feedback", C.c_int32), ("max_heavy", C.c_int32),
-                ("heavy_bytes", C.c_uint64), ("max_batch", C.c_int32), ("batch_ratio", C.c_int32)]
+                ("heavy_bytes", C.c_uint64), ("max_batch", C.c_int32), ("batch_ratio", C.c_int32),
+                ("batch_min_share", C.c_double)]
 
 
 class ServeRequest(C.Structure):

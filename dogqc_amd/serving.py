@@ -34,12 +34,13 @@ class Server(object):
     """ctypes wrapper of one dqc_serve server."""
 
     def __init__(self, slots, arena_bytes, max_retries=8, bypass_ms=50, size_feedback=True, max_heavy=0,
-                 heavy_bytes=0, max_batch=0, batch_ratio=0):
+                 heavy_bytes=0, max_batch=0, batch_ratio=0, batch_min_share=0.0):
         cfg = rt.ServeConfig()
         cfg.slots, cfg.max_retries, cfg.arena_bytes, cfg.bypass_ms = slots, max_retries, int(arena_bytes), bypass_ms
         cfg.size_feedback = 1 if size_feedback else 0
         cfg.max_heavy, cfg.heavy_bytes = int(max_heavy), int(heavy_bytes)
         cfg.max_batch, cfg.batch_ratio = int(max_batch), int(batch_ratio)
+        cfg.batch_min_share = float(batch_min_share)
         self.handle = C.c_void_p()
         self._alive = []      # host arrays the server reads after submit
         rt.check(rt.lib().dqc_serve_create(C.byref(cfg), C.byref(self.handle)), "dqc_serve_create")
@@ -194,15 +195,17 @@ def poissonArrivals(queries, qps, seed=5):
 
 
 def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600, log=None, max_heavy=0,
-        heavy_bytes=0, max_batch=0, batch_ratio=0):
+        heavy_bytes=0, max_batch=0, batch_ratio=0, batch_min_share=0.0):
     """Run queries through a server with `slots` slots; returns (records,
     server) -- the server stays alive so kept results can be read.  With
     `max_batch` > 1, waiting queries of one config run in batches of up to
     that many (batched plans, dogqc_amd/batching.py) whose driving row
-    counts are within `batch_ratio` of each other (0: any).  With `log`, reports
+    counts are within `batch_ratio` of each other (0: any); a batch's largest
+    member runs on its own when the others hold less than `batch_min_share`
+    of its driving rows (0: never).  With `log`, reports
     every 5 s how many queries have finished."""
     srv = Server(slots, arena_bytes, max_heavy=max_heavy, heavy_bytes=heavy_bytes, max_batch=max_batch,
-                 batch_ratio=batch_ratio)
+                 batch_ratio=batch_ratio, batch_min_share=batch_min_share)
     ids = []
     bindings = {}
     for i, q in enumerate(queries):
@@ -367,7 +370,7 @@ def matchesOracle(config, got, want):
 
 
 def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None,
-              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=8):
+              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=8, batch_min_share=2.0):
     """Serving benchmark on resident tables at `scale` x the base sizes.
     Returns a dict of capacity (serial / batched QPS), latency at a fixed
     Poisson rate (load x the serial capacity) and verification results."""
@@ -390,7 +393,7 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
         sample += [i for i in order if queries[i].config == c][:max(1, verify // len(CONFIG_PLANS))]
     out = {"scale": scale, "resident_gb": tables.nbytes() / 1e9, "queries": nqueries, "slots": slots,
            "arena_gb": arena_gb, "max_heavy": max_heavy, "heavy_mb": heavy_mb, "max_batch": max_batch,
-           "batch_ratio": batch_ratio, "plans": dict(plans.names)}
+           "batch_ratio": batch_ratio, "batch_min_share": batch_min_share, "plans": dict(plans.names)}
     # capacity: every query arrives at t = 0
     for q in queries:
         q.arrival_ns = 0
@@ -410,7 +413,7 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
             list(L.split_capacity[:L.nhts])))
     say("serial capacity: %.1f q/s" % out["serial"]["qps"])
     hv = {"max_heavy": max_heavy, "heavy_bytes": int(heavy_mb * (1 << 20)), "max_batch": max_batch,
-          "batch_ratio": batch_ratio}
+          "batch_ratio": batch_ratio, "batch_min_share": batch_min_share}
     recsB, srvB = run(tables, plans, queries, slots, arena, keep=set(sample), log=say, **hv)
     out["batched"] = latencyStats(recsB)
     out["batched_by_config"] = perConfig(queries, recsB)

@@ -50,9 +50,10 @@ def test_datagen_header_and_exports():
 
 def test_serve_structs_match_header():
     # slots, max_retries, arena_bytes, bypass_ms, size_feedback, max_heavy, 4 bytes of
-    # padding before heavy_bytes, max_batch, batch_ratio
-    assert ctypes.sizeof(rt.ServeConfig) == 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8 + 4 * 2 == 48
+    # padding before heavy_bytes, max_batch, batch_ratio, batch_min_share
+    assert ctypes.sizeof(rt.ServeConfig) == 4 + 4 + 8 + 4 + 4 + 4 + 4 + 8 + 4 * 2 + 8 == 56
     assert rt.ServeConfig.max_batch.offset == 40 and rt.ServeConfig.batch_ratio.offset == 44
+    assert rt.ServeConfig.batch_min_share.offset == 48
     assert rt.ServeConfig.heavy_bytes.offset == 32
     assert ctypes.sizeof(rt.ServeRequest) == 8 + 24 + 8 + 8 + 4 + 4 + 8 * 4 + 4 + 4 == 96
     assert rt.ServeRequest.batch_module.offset == 56

@@ -184,6 +184,35 @@ def test_batched_queries_equal_alone_and_oracle():
 
 
 
+def test_batch_min_share_runs_a_dominant_query_alone():
+    """With batch_min_share, a batch runs only if its members other than the
+    largest hold at least that share of the largest's driving rows; results
+    stay the same."""
+    print("gpu case: serving batch min share", flush=True)
+    T = tables(0.02)
+    plans = serving.PlanSet()
+    queries = serving.makeMix(32, seed=17, scaleRange=(0.0005, 0.02))
+    keep = set(range(len(queries)))
+    recsA, srvA = serving.run(T, plans, queries, 1, 4 << 30, keep=keep)
+    recsB, srvB = serving.run(T, plans, queries, 4, 4 << 30, keep=keep, max_batch=8, batch_min_share=1.0)
+    driving = [max(serving.prefixRows(T, q.config, q.scale).values()) for q in queries]
+    batches = {}
+    for i, r in enumerate(recsB):
+        if r.batch_size > 1:
+            batches.setdefault((r.slot, r.admit_ns), []).append(i)
+    assert batches, "no batch formed"
+    for members in batches.values():
+        rows = sorted(driving[i] for i in members)
+        assert sum(rows[:-1]) >= rows[-1], [(queries[i].config, queries[i].scale) for i in members]
+    for i, q in enumerate(queries):
+        assert serving.sameResult(q.config, serving.resultOf(plans, q.config, recsA[i]),
+                                  serving.resultOf(plans, q.config, recsB[i])), "query %d alone != batched" % i
+    for i in range(len(queries)):
+        srvB.release(i)
+    srvA.destroy()
+    srvB.destroy()
+
+
 def test_batch_ratio_keeps_batches_of_similar_size():
     """With batch_ratio, a batch holds only queries whose driving row counts
     are within that factor of its first query's; results stay the same."""
