@@ -228,6 +228,13 @@ def main():
     print(json.dumps(out))
 
 
+def servingCheck():
+    """The checker of the served sample: the C oracle over host copies of
+    each sampled query's row prefixes, called after the timed runs."""
+    from oracle import servecheck
+    return servecheck.check
+
+
 def benchServing(args):
     """c4: queries/s of the serving loop (all queries arriving at once) with
     `slots` concurrent queries against one at a time, and p50/p99 latency of
@@ -240,7 +247,8 @@ def benchServing(args):
     log = lambda m: print("[c4] " + m, file=sys.stderr, flush=True)
     r = serving.benchmark(scale=args.scale, nqueries=nq, slots=args.slots, arena_gb=args.arena_gb, log=log,
                           max_heavy=args.max_heavy, heavy_mb=args.heavy_mb, max_batch=args.max_batch,
-                          batch_ratio=args.batch_ratio, batch_min_share=args.batch_min_share)
+                          batch_ratio=args.batch_ratio, batch_min_share=args.batch_min_share,
+                          check=servingCheck())
     out = {
         "metric": METRIC,
         "value": r["batched"]["qps"],

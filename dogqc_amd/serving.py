@@ -13,7 +13,8 @@ baseline on the same code path.
 ``benchmark()`` measures both: capacity (all queries arrive at once,
 queries per second over the makespan) and latency under a Poisson stream at
 a fixed rate (p50/p99 of arrival-to-completion), and checks a sample of the
-results against the CPU oracle and between the alone and mixed runs.
+results between the alone and mixed runs (and, through a caller-supplied
+check, against an independent result).
 """
 import ctypes as C
 import json
@@ -333,47 +334,13 @@ def sameResult(config, a, b, rtol=1e-9):
     return True
 
 
-def oracleResult(tables, config, scale):
-    """The CPU oracle's result of one query (host copies of the prefixes)."""
-    from oracle import coracle
-    rows = prefixRows(tables, config, scale)
-    acc = tables.hostAccessor(rows)
-    cols = {t: {a: acc.column(t, a) for a in acc.schema[t]["attributes"]} for t in rows}
-    if config == "c0":
-        ref = coracle.c0(cols["cfg0"])
-        out = OrderedDict((k, []) for k in ("g1", "g2", "sum_qty", "sum_base_price", "sum_disc_price",
-                                             "sum_charge", "count_order"))
-        for (g1, g2), v in sorted(ref.items()):
-            for k, x in zip(out, (g1, g2) + tuple(v[:4]) + (v[5],)):
-                out[k].append(x)
-        return OrderedDict((k, np.array(v)) for k, v in out.items())
-    if config == "c1":
-        keys, sums = coracle.c1(cols["dim"], cols["fact"])
-        return OrderedDict([("d_key", keys), ("revenue", sums)])
-    if config == "c2":
-        cnt, s = coracle.c2(cols["build"], cols["probe"])
-        return OrderedDict([("cnt", np.array([cnt], dtype=np.int32)), ("sum_prod", np.array([s], dtype=np.int64))])
-    return coracle.c3(cols, workloads.C3_WORD)
-
-
-def matchesOracle(config, got, want):
-    if config == "c0":
-        sub = OrderedDict((k, np.asarray(got[k])) for k in want)
-        sub["count_order"] = sub["count_order"].astype(np.int64)
-        want = OrderedDict(want)
-        want["count_order"] = want["count_order"].astype(np.int64)
-        want["g1"], want["g2"] = want["g1"].astype(np.int8), want["g2"].astype(np.int8)
-        return sameResult("c0", sub, want)
-    if config == "c2":
-        return int(got["cnt"][0]) == int(want["cnt"][0]) and int(got["sum_prod"][0]) == int(want["sum_prod"][0])
-    return sameResult(config, got, want)
-
-
 def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None,
-              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=8, batch_min_share=2.0):
+              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=8, batch_min_share=2.0, check=None):
     """Serving benchmark on resident tables at `scale` x the base sizes.
     Returns a dict of capacity (serial / batched QPS), latency at a fixed
-    Poisson rate (load x the serial capacity) and verification results."""
+    Poisson rate (load x the serial capacity) and verification results.
+    `check(tables, config, scale, result) -> bool`, when given, is called
+    after the timed runs on each result of the verified sample."""
     from dogqc_amd import datagen
     say = log or (lambda *a: None)
     t0 = time.perf_counter()
@@ -386,7 +353,7 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     say("modules loaded and warmed up")
     queries = makeMix(nqueries, seed)
     # the verified sample: the smallest queries of each config, so the CPU
-    # oracle over host copies of their prefixes stays within seconds
+    # check over host copies of their prefixes stays within seconds
     order = sorted(range(nqueries), key=lambda i: queries[i].scale)
     sample = []
     for c in CONFIG_PLANS:
@@ -427,19 +394,22 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     out["batched"]["in_batches"] = len(sizes)
     out["batched"]["mean_batch"] = float(np.mean(sizes)) if sizes else 1.0
     say("batched capacity: %.1f q/s" % out["batched"]["qps"])
-    # alone vs mixed, and the CPU oracle, on the sample
-    alone_same, oracle_ok = 0, 0
+    # alone vs mixed, and the caller's check, on the sample
+    alone_same, checked_ok = 0, 0
     for i in sample:
         q = queries[i]
         a = resultOf(plans, q.config, recsS[i])
         b = resultOf(plans, q.config, recsB[i])
         alone_same += sameResult(q.config, a, b)
-        oracle_ok += matchesOracle(q.config, b, oracleResult(tables, q.config, q.scale))
-    out["verify"] = {"sample": len(sample), "alone_equals_mixed": alone_same, "oracle_ok": oracle_ok,
-                     "ok": alone_same == len(sample) and oracle_ok == len(sample)}
+        if check is not None:
+            checked_ok += bool(check(tables, q.config, q.scale, b))
+    out["verify"] = {"sample": len(sample), "alone_equals_mixed": alone_same,
+                     "checked_ok": checked_ok if check is not None else None,
+                     "ok": alone_same == len(sample) and (check is None or checked_ok == len(sample))}
     srvS.destroy()
     srvB.destroy()
-    say("verified %d queries: alone == mixed %d, oracle %d" % (len(sample), alone_same, oracle_ok))
+    say("verified %d queries: alone == mixed %d, checked %s" % (len(sample), alone_same,
+                                                                 checked_ok if check is not None else "-"))
     # latency at a fixed offered rate
     rate = load * out["serial"]["qps"]
     poissonArrivals(queries, rate, seed + 1)

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from dogqc_amd import datagen, runtime as rt, serving, workloads
+from oracle import servecheck
 
 pytestmark = pytest.mark.gpu
 
@@ -106,8 +107,8 @@ def test_alone_equals_mixed_and_oracle():
         a = serving.resultOf(plans, q.config, recsA[i])
         m = serving.resultOf(plans, q.config, recsM[i])
         assert serving.sameResult(q.config, a, m), "query %d %s x%.4f alone != mixed" % (i, q.config, q.scale)
-        want = serving.oracleResult(T, q.config, q.scale)
-        assert serving.matchesOracle(q.config, m, want), "query %d %s x%.4f != oracle" % (i, q.config, q.scale)
+        want = servecheck.oracleResult(T, q.config, q.scale)
+        assert servecheck.matchesOracle(q.config, m, want), "query %d %s x%.4f != oracle" % (i, q.config, q.scale)
         overlap += sum(1 for j, r in enumerate(recsM) if j != i and r.admit_ns < recsM[i].done_ns
                        and recsM[i].admit_ns < r.done_ns)
     assert overlap > 0          # the mixed run did run queries concurrently
@@ -153,7 +154,7 @@ def test_overflow_is_readmitted():
     recs, srv = serving.run(T, plans, [q], 2, 1 << 30, keep={0})
     assert recs[0].retries > 0
     got = serving.resultOf(plans, "c1", recs[0])
-    assert serving.matchesOracle("c1", got, serving.oracleResult(T, "c1", 0.02))
+    assert servecheck.matchesOracle("c1", got, servecheck.oracleResult(T, "c1", 0.02))
     srv.destroy()
 
 
@@ -175,7 +176,7 @@ def test_batched_queries_equal_alone_and_oracle():
         b = serving.resultOf(plans, q.config, recsB[i])
         assert serving.sameResult(q.config, a, b), "query %d %s x%.4f alone != batched (batch %d)" % (
             i, q.config, q.scale, recsB[i].batch_size)
-        assert serving.matchesOracle(q.config, b, serving.oracleResult(T, q.config, q.scale)), \
+        assert servecheck.matchesOracle(q.config, b, servecheck.oracleResult(T, q.config, q.scale)), \
             "query %d %s x%.4f batched != oracle" % (i, q.config, q.scale)
     for i in range(len(queries)):
         srvB.release(i)
