@@ -156,7 +156,11 @@ class BruteForce(object):
         res = OrderedDict()
         for id, a in node.outRelation.items():
             vals = [r[id] for r in rows]
-            res[a.name] = np.array(vals, dtype=object) if a.dataType == Type.STRING else np.array(vals)
+            if a.dataType == Type.STRING:
+                res[a.name] = np.array(vals, dtype=object)
+            else:
+                # an empty result keeps its column's register type
+                res[a.name] = np.array(vals) if vals else np.zeros(0, dtype=regType(a.dataType, self.decimal))
         return res
 
 

@@ -52,6 +52,21 @@ def _sortKey(cols, names):
     return np.lexsort(keys) if keys else np.arange(0)
 
 
+def _widen(g, w):
+    """Both arrays in one type wide enough for either's values: the wider
+    float, or the wider integer (int64 for a signed/unsigned mix, which the
+    result columns' value ranges fit)."""
+    if g.dtype == w.dtype:
+        return g, w
+    if g.dtype.kind == "f" or w.dtype.kind == "f":
+        t = np.float64 if max(g.itemsize, w.itemsize) == 8 else np.float32
+        return g.astype(t), w.astype(t)
+    t = np.result_type(g.dtype, w.dtype)
+    if t.kind == "f":
+        t = np.int64
+    return g.astype(t), w.astype(t)
+
+
 def assertSameResult(got, want, tolCols=(), rtol=1e-9, atol=0.0, label=""):
     assert list(got) == list(want), "%s: columns %s != %s" % (label, list(got), list(want))
     n = len(next(iter(want.values()))) if want else 0
@@ -68,11 +83,16 @@ def assertSameResult(got, want, tolCols=(), rtol=1e-9, atol=0.0, label=""):
                 label, k, rtol, int(bad.sum()), g[bad][:3], w[bad][:3])
         elif w.dtype == object:
             assert list(g) == list(w), "%s: column %s differs" % (label, k)
-        elif w.dtype.kind == "f":
-            same = (g.view(np.uint64 if g.itemsize == 8 else np.uint32) ==
-                    w.astype(g.dtype).view(np.uint64 if g.itemsize == 8 else np.uint32)) | (np.isnan(g) & np.isnan(w))
-            assert same.all(), "%s: column %s not bit-identical at %d rows, e.g. %r vs %r" % (
-                label, k, int((~same).sum()), g[~same][:3], w[~same][:3])
         else:
-            assert np.array_equal(g, w.astype(g.dtype)), "%s: column %s differs at %d rows" % (
-                label, k, int((g != w.astype(g.dtype)).sum()))
+            # exact columns compare at the wider of the two types, so a GPU
+            # column that wrapped or rounded into a narrower type fails
+            assert g.dtype.itemsize >= w.dtype.itemsize, "%s: column %s is %s on the GPU, %s in the oracle" % (
+                label, k, g.dtype, w.dtype)
+            g, w = _widen(g, w)
+            if g.dtype.kind == "f":
+                bits = np.uint64 if g.itemsize == 8 else np.uint32
+                same = (g.view(bits) == w.view(bits)) | (np.isnan(g) & np.isnan(w))
+                assert same.all(), "%s: column %s not bit-identical at %d rows, e.g. %r vs %r" % (
+                    label, k, int((~same).sum()), g[~same][:3], w[~same][:3])
+            else:
+                assert np.array_equal(g, w), "%s: column %s differs at %d rows" % (label, k, int((g != w).sum()))
