@@ -66,6 +66,9 @@ class LaneRefill(UnaryTranslator):
             pipe.emit("%s[wave][rpos] = %s[u];" % (q, r))
         pipe.code.close()
         pipe.emit("%s += dqc::popc64(rm);" % cnt)
+        # entries written by some lanes are read back by others: order the
+        # LDS stores before the pops below
+        pipe.emit("dqc::wave_lds_sync();")
         pipe.code.close()
         if outer is None:
             pipe.emit("const bool rflush%d = !more;" % X)
@@ -82,6 +85,8 @@ class LaneRefill(UnaryTranslator):
         for r, q in queues:
             pipe.emit("%s[u] = %s[wave][%s + lane];" % (r, q, cnt))
         pipe.code.close()
+        # the popped entries are read before later pushes reuse their slots
+        pipe.emit("dqc::wave_lds_sync();")
         point["cond"] = "%s >= (rflush%d ? 1u : %s)" % (cnt, X, thr)
         self.parent.consume(ctxt)
         pipe.code.close()
