@@ -638,29 +638,57 @@ DQC_DEVICE_INLINE str_t string_substring(str_t s, int from, int len) {
     return r;
 }
 
-// SQL LIKE with '%' (any run) and '_' (any one byte): iterative wildcard
-// matching that backtracks to the most recent '%' (textbook greedy method).
+// LIKE with the semantics of stringLikeCheck (util.cuh:148-199), which is
+// not textbook LIKE: the literal prefix (before the first '%') and suffix
+// (after the last '%') are compared independently, each only as far as the
+// string reaches (so they may overlap: 'ab' LIKE 'ab%b' and 'a' LIKE 'a_'
+// hold, and a pattern without '%' holds for every string that starts and
+// ends with it); the segments between the first and last '%' are then found
+// left to right between prefix and suffix, an empty segment ('%%') matching
+// only a literal '%' byte.  '_' matches any byte.  Every read is bounds
+// checked: the string byte the reference reads at the end of the searched
+// region cannot change the result, and pattern bytes past its end read as 0
+// (the terminator of the pattern literal there).
 DQC_DEVICE_INLINE bool string_like(str_t s, str_t p) {
-    const char* si = s.start;
-    const char* pi = p.start;
-    const char* star = nullptr;
-    const char* mark = nullptr;
-    while (si < s.end) {
-        if (pi < p.end && (*pi == '_' || *pi == *si)) {
-            ++si;
-            ++pi;
-        } else if (pi < p.end && *pi == '%') {
-            star = pi++;
-            mark = si;
-        } else if (star) {
-            pi = star + 1;
-            si = ++mark;
-        } else {
-            return false;
+    const int64_t n = s.end - s.start;
+    const int64_t m = p.end - p.start;
+    auto pc = [&](int64_t i) -> char { return (i >= 0 && i < m) ? p.start[i] : '\0'; };
+    auto cmp = [](char c, char l) { return c == l || l == '_'; };
+    int64_t lPos = 0, sPos = 0;
+    int64_t lInStart = 0, lInEnd = m, sInStart = 0, sInEnd = n;
+    if (pc(0) != '%') {  // prefix
+        for (; lPos < m && sPos < n && pc(lPos) != '%'; ++lPos, ++sPos)
+            if (!cmp(s.start[sPos], pc(lPos))) return false;
+        lInStart = lPos;
+        sInStart = sPos;
+    }
+    if (m == 0) return true;  // the empty pattern matches every string
+    if (pc(m - 1) != '%') {  // suffix
+        sPos = n - 1;
+        lPos = m - 1;
+        for (; lPos >= 0 && sPos >= 0 && pc(lPos) != '%'; --lPos, --sPos)
+            if (!cmp(s.start[sPos], pc(lPos))) return false;
+        lInEnd = lPos;
+        sInEnd = sPos + 1;
+    }
+    if (lInStart < lInEnd) {  // infix segments, left to right
+        lPos = lInStart + 1;
+        sPos = sInStart;
+        while (sPos < sInEnd && lPos < lInEnd) {
+            int64_t lTrace = lPos, sTrace = sPos;
+            while (sTrace < sInEnd && cmp(s.start[sTrace], pc(lTrace))) {
+                ++lTrace;
+                if (pc(lTrace) == '%') {
+                    lPos = ++lTrace;
+                    sPos = sTrace;
+                    break;
+                }
+                ++sTrace;
+            }
+            ++sPos;
         }
     }
-    while (pi < p.end && *pi == '%') ++pi;
-    return pi == p.end;
+    return lPos >= lInEnd;
 }
 
 // Batch segment of row `row`: the first member k with row < prefix[k] (the
@@ -688,12 +716,14 @@ DQC_DEVICE_INLINE int32_t segment_of(const int64_t* prefix, int64_t members, int
     return s0 == s1 ? s0 : segment_of(prefix, members, row);
 }
 
-// LIKE patterns made of literals and '%' only (no '_') are matched without
-// the general backtracking matcher: a prefix and a suffix compare bytes in
-// place, and each inner literal is found left to right by string_find, which
-// reads the string with aligned 8-byte loads and slides a register window
-// over its bytes (an aligned word holding a byte of the string lies in the
-// same page as that byte, so the loads stay in mapped memory).
+// LIKE patterns made of literals and '%' only (no '_'), holding a '%' and no
+// "%%", have generated matchers.  For a string at least as long as the
+// prefix and suffix together, string_like's answer is the textbook one: the
+// prefix and suffix compare bytes in place, and each inner literal is found
+// left to right by string_find, which reads the string with aligned 8-byte
+// loads and slides a register window over its bytes (an aligned word holding
+// a byte of the string lies in the same page as that byte, so the loads stay
+// in mapped memory).  Shorter strings take string_like.
 
 DQC_DEVICE_INLINE bool bytes_equal(const char* a, const char* b, uint32_t n) {
     for (uint32_t k = 0; k < n; ++k)

@@ -135,26 +135,31 @@ class ExprEmitter(object):
 
 
 def likeMatcherDecl(name, pattern, lit):
-    """Device function matching LIKE `pattern` (literals and '%' only):
-    the first segment is a prefix, the last a suffix (unless the pattern
-    starts / ends with '%'), the others are found in order in between;
-    `lit(text)` names the device constant holding a literal."""
+    """Device function matching LIKE `pattern` (literals and '%' only) with
+    dqc::string_like's semantics (util.cuh:148-199).  When the pattern holds
+    a '%' and no "%%", a string of at least P + S bytes (P before the first
+    '%', S after the last) has the textbook answer: the first segment is a
+    prefix, the last a suffix (unless the pattern starts / ends with '%'),
+    the others are found in order in between.  Shorter strings, and other
+    patterns, call dqc::string_like.  `lit(text)` names the device constant
+    holding a literal."""
     segs = pattern.split("%")
     body = ["static __device__ __forceinline__ bool %s(dqc::str_t s) {" % name]
-    if len(segs) == 1:
-        body.append("    return dqc::string_equals(s, %s);" % lit(pattern)[0])
+    if len(segs) == 1 or "%%" in pattern:
+        body.append("    return dqc::string_like(s, %s);" % lit(pattern)[0])
         body.append("}")
         return "\n".join(body)
-    pre, mid, suf = segs[0], [m for m in segs[1:-1] if m], segs[-1]
+    pre, mid, suf = segs[0], segs[1:-1], segs[-1]
     n = lambda t: len(t.encode("latin-1"))
     body.append("    const char* pos = s.start;")
     body.append("    const char* lim = s.end;")
+    if pre or suf:
+        body.append("    if (s.end - s.start < %d) return dqc::string_like(s, %s);"
+                    % (n(pre) + n(suf), lit(pattern)[0]))
     if suf:
-        body.append("    if (s.end - s.start < %d) return false;" % (n(pre) + n(suf)))
         body.append("    lim = s.end - %d;" % n(suf))
         body.append("    if (!dqc::bytes_equal(lim, %s, %du)) return false;" % (lit(suf)[1], n(suf)))
     if pre:
-        body.append("    if (lim - pos < %d) return false;" % n(pre))
         body.append("    if (!dqc::bytes_equal(pos, %s, %du)) return false;" % (lit(pre)[1], n(pre)))
         body.append("    pos += %d;" % n(pre))
     for m in mid:

@@ -7,6 +7,7 @@ entry ahead of time so no compilation happens on the GPU machine.
 """
 from collections import OrderedDict
 import contextlib
+import itertools
 
 import numpy as np
 
@@ -277,6 +278,91 @@ def _featurePlan(name, alg):
     raise KeyError(name)
 
 
+# ------------------------------------------------------------- string plans
+# Every string over {a, b, c} of at most 5 bytes, and every string over
+# {a, %, _} of at most 3 bytes, each 4 times with SUBSTRING arguments in
+# [-2, 7]; LIKE patterns over {a, b, %, _} of at most 3 bytes and longer ones
+# of the shapes the matcher generator distinguishes.
+
+STRING_SCHEMA = OrderedDict([
+    ("w", OrderedDict([("w_id", Type.INT), ("w_s", Type.STRING), ("w_f", Type.INT), ("w_l", Type.INT)])),
+])
+
+
+def _allStrings(alphabet, maxLen):
+    out = [b""]
+    for n in range(1, maxLen + 1):
+        out += [bytes(t) for t in itertools.product(alphabet, repeat=n)]
+    return out
+
+
+LIKE_PATTERNS = [p.decode() for p in _allStrings(b"ab%_", 3)[1:]] + [
+    "", "ab%b", "a%%c", "a%%", "%%c", "%%", "%%%", "abab", "%abc", "abc%", "a%b%c", "%a%b%", "%ab%ba%",
+    "_%_", "%_a_%", "ab%_b", "a_%_c", "%c%", "cab%", "%bca", "a%a%a", "%aaa%", "b%%a", "%a%%b%", "_b_",
+    "abcab", "%bcab", "ab%ab", "%%a%", "c%c%", "%%ab%%"]
+
+
+def stringData(copies=4, seed=11):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    strs = (_allStrings(b"abc", 5) + _allStrings(b"a%_", 3)[1:]) * copies
+    n = len(strs)
+    w = OrderedDict()
+    w["w_id"] = (Type.INT, np.arange(n, dtype=np.int32))
+    w["w_s"] = (Type.STRING, strs)
+    w["w_f"] = (Type.INT, rng.integers(-2, 8, size=n).astype(np.int32))
+    w["w_l"] = (Type.INT, rng.integers(-2, 8, size=n).astype(np.int32))
+    return _asAccessor(OrderedDict([("w", w)]))
+
+
+def stringRows(acc, n):
+    """The first n rows of a string table (an empty or one-row case)."""
+    offs, chars = acc.column("w", "w_s")
+    strs = [bytes(chars[offs[i]:offs[i + 1]]) for i in range(n)]
+    w = OrderedDict()
+    w["w_id"] = (Type.INT, np.asarray(acc.column("w", "w_id"))[:n])
+    w["w_s"] = (Type.STRING, strs)
+    w["w_f"] = (Type.INT, np.asarray(acc.column("w", "w_f"))[:n])
+    w["w_l"] = (Type.INT, np.asarray(acc.column("w", "w_l"))[:n])
+    return _asAccessor(OrderedDict([("w", w)]))
+
+
+def _stringPlan(name, alg):
+    if name == "s_like":
+        # bit k of column b<j> = w_s LIKE LIKE_PATTERNS[16 j + k]
+        node = alg.scan("w")
+        cols = ["w_id"]
+        for j in range(0, len(LIKE_PATTERNS), 16):
+            expr = None
+            for k, pat in enumerate(LIKE_PATTERNS[j:j + 16]):
+                term = scal.CaseExpr([(scal.LikeExpr(_A("w_s"), _C(pat, Type.STRING)), _C(str(1 << k), Type.INT))],
+                                     _C("0", Type.INT))
+                expr = term if expr is None else scal.AddExpr(expr, term)
+            node = alg.map("b%d" % (j // 16), expr, node)
+            cols.append("b%d" % (j // 16))
+        return alg.projection(cols, node), {}
+    if name == "s_substr":
+        # per-row SUBSTRING arguments; equality and IN on the substring
+        node = alg.map("sub", scal.SubstringExpr(_A("w_s"), _A("w_f"), _A("w_l")), alg.scan("w"))
+        node = alg.map("eq", scal.CaseExpr([(scal.EqualsExpr(_A("sub"), _C("ab", Type.STRING)), _C("1", Type.INT))],
+                                           _C("0", Type.INT)), node)
+        node = alg.map("inn", scal.CaseExpr([(scal.InExpr(_A("sub"), [_C(t, Type.STRING) for t in
+                                                                       ("", "a", "%_", "cab")]),
+                                              _C("1", Type.INT))], _C("0", Type.INT)), node)
+        return alg.projection(["w_id", "sub", "eq", "inn"], node), {}
+    if name == "s_filter":
+        # string predicates as a filter: IN, NOT, OR, equality of two strings
+        cond = scal.OrExpr(scal.InExpr(_A("w_s"), [_C(t, Type.STRING) for t in ("abc", "", "a%")]),
+                           scal.AndExpr(scal.NotExpr(scal.LikeExpr(_A("w_s"), _C("%b%", Type.STRING))),
+                                        scal.EqualsExpr(scal.SubstringExpr(_A("w_s"), _C("1", Type.INT),
+                                                                           _C("2", Type.INT)),
+                                                        scal.SubstringExpr(_A("w_s"), _C("2", Type.INT),
+                                                                           _C("2", Type.INT)))))
+        return alg.projection(["w_id", "w_s"], alg.selection(cond, alg.scan("w"))), {2: {"selectivity": 0.2}}
+    raise KeyError(name)
+
+
+STRING_PLANS = ["s_like", "s_substr", "s_filter"]
+
 FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_project", "f_join_unique",
                  "f_join_unique_lowest",
                  "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
@@ -297,6 +383,8 @@ PLANS["c3_refill"] = ("c3", {"buffers": [6]})
 PLANS["cal"] = ("cal", {})
 for _n in FEATURE_PLANS:
     PLANS[_n] = ("feature", {})
+for _n in STRING_PLANS:
+    PLANS[_n] = ("strings", {})
 PLANS["f_join_multi_pushdown"] = ("feature", {"pushdown": True, "base": "f_join_multi"})
 PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_project"})
 
@@ -341,6 +429,10 @@ def build(name, acc):
         alg = RelationalAlgebra(acc)
         p, cfg = _featurePlan(opts.get("base", name), alg)
         return alg, p, cfg, opts
+    if kind == "strings":
+        alg = RelationalAlgebra(acc)
+        p, cfg = _stringPlan(name, alg)
+        return alg, p, cfg, opts
     alg, p, cfg = workloads.plan(kind, acc)
     return alg, p, cfg, opts
 
@@ -355,6 +447,8 @@ def emptyAccessor(name):
         cfgName = PLANS[name][1]["config"]
         return batching.withBatch(workloads.emptyAccessor(cfgName),
                                   {t: np.zeros(1, dtype=np.int64) for t in batching.SCALED[cfgName]})
+    if kind == "strings":
+        return stringRows(stringData(1), 0)
     return featureEmpty() if kind == "feature" else workloads.emptyAccessor(kind)
 
 

@@ -25,8 +25,8 @@ It restates the reference's operator semantics operator-at-a-time:
     the type's maximum and MAX at its lowest value;
   * a temp table (createtemp, relationalAlgebra.py Materialize.temp) holds
     its child's rows; a later root's scan of it reads them by attribute name;
-  * SQL LIKE follows util.cuh:144-199 for '%' and '_' patterns (via an
-    equivalent regular expression), SUBSTRING util.cuh:132-142.
+  * LIKE, SUBSTRING and string equality are strfuncs.py's restatements of
+    util.cuh:90-102, 132-199 (the reference's LIKE, not textbook LIKE).
 The result is an OrderedDict attribute name -> numpy array (strings as
 object arrays of bytes), in the plan's output attribute order.
 """
@@ -40,6 +40,7 @@ import dogqc_amd.relationalAlgebra as ra
 import dogqc_amd.scalarAlgebra as scal
 from dogqc_amd.relationalAlgebra import Reduction, Join
 from dogqc_amd.types import Type
+from oracle import strfuncs
 
 FP64 = "double"
 
@@ -108,18 +109,6 @@ def _cast(v, ct):
     return np.array(v).astype(ct)[()]
 
 
-def _likeRegex(pattern):
-    out = []
-    for ch in pattern.decode("latin-1"):
-        if ch == "%":
-            out.append(".*")
-        elif ch == "_":
-            out.append(".")
-        else:
-            out.append(re.escape(ch))
-    return re.compile("".join(out), re.DOTALL)
-
-
 class Evaluator(object):
 
     def __init__(self, accessor, decimal=FP64):
@@ -145,9 +134,9 @@ class Evaluator(object):
         if isinstance(e, scal.LikeExpr):
             a, _ = self.expr(e.leftChild, rel, n)
             p, _ = self.expr(e.rightChild, rel, n)
-            rx = _likeRegex(p)
-            return np.fromiter((rx.fullmatch(s.decode("latin-1")) is not None for s in a), dtype=bool,
-                               count=len(a)), "bool"
+            if isinstance(p, bytes):
+                return strfuncs.likeColumn(a, p), "bool"
+            return np.fromiter((strfuncs.like(x, y) for x, y in zip(a, p)), dtype=bool, count=len(a)), "bool"
         if isinstance(e, (scal.EqualsExpr, scal.SmallerExpr, scal.SmallerEqualExpr, scal.LargerExpr,
                           scal.LargerEqualExpr)):
             a, at = self.expr(e.leftChild, rel, n)
@@ -155,9 +144,11 @@ class Evaluator(object):
             if at == "str" or bt == "str":
                 if not isinstance(e, scal.EqualsExpr):
                     raise NotImplementedError("string ordering")
+                if isinstance(a, bytes):
+                    a, b = b, a
                 if isinstance(b, bytes):
-                    return np.fromiter((s == b for s in a), dtype=bool, count=len(a)), "bool"
-                return np.fromiter((x == y for x, y in zip(a, b)), dtype=bool, count=len(a)), "bool"
+                    return np.fromiter((strfuncs.equals(x, b) for x in a), dtype=bool, count=len(a)), "bool"
+                return np.fromiter((strfuncs.equals(x, y) for x, y in zip(a, b)), dtype=bool, count=len(a)), "bool"
             ct = _arith(at, bt)
             a, b = _cast(a, ct), _cast(b, ct)
             op = {scal.EqualsExpr: np.equal, scal.SmallerExpr: np.less, scal.SmallerEqualExpr: np.less_equal,
@@ -213,10 +204,7 @@ class Evaluator(object):
             ln = np.broadcast_to(np.asarray(ln, dtype=np.int64), (len(s),))
             out = np.empty(len(s), dtype=object)
             for i, x in enumerate(s):
-                st = int(f[i]) - 1
-                en = st + int(ln[i])
-                st, en = max(st, 0), min(en, len(x))
-                out[i] = x[st:en] if st < en else b""
+                out[i] = strfuncs.substring(x, int(f[i]), int(ln[i]))
             return out, "str"
         if isinstance(e, scal.CaseExpr):
             ct = regType(Type.FLOAT, self.decimal)

@@ -278,6 +278,17 @@ def test_feature_plans(name, size):
     check(name, acc, label="t=%d u=%d" % size)
 
 
+@pytest.mark.parametrize("name", planlib.STRING_PLANS)
+@pytest.mark.parametrize("rows", [None, 1, 0])
+def test_string_plans(name, rows):
+    """LIKE (the reference's semantics, util.cuh:148-199), SUBSTRING, IN and
+    string equality on every short string against 115 patterns, per row."""
+    acc = planlib.stringData()
+    if rows is not None:
+        acc = planlib.stringRows(acc, rows)
+    check(name, acc, label="rows=%s" % rows)
+
+
 def test_feature_retry_all_buffers():
     acc = planlib.featureData()
     t = rt.newTuning()

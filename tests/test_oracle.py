@@ -53,6 +53,8 @@ def test_planeval_matches_bruteforce(name):
     kind = planlib.PLANS[name][0]
     if kind == "feature":
         acc = planlib.featureData(160, 40, seed=3)
+    elif kind == "strings":
+        acc = planlib.stringRows(planlib.stringData(1), 120)
     elif kind == "tpch":
         acc = tpchAccessor()
     elif kind == "batch":
@@ -101,6 +103,12 @@ CASES = [
     ("char_eq", scal.EqualsExpr(A("c"), K("R", Type.CHAR))),
     ("case", scal.CaseExpr([(scal.LargerExpr(A("x"), K("1.0", Type.DOUBLE)), A("i"))], A("f"))),
     ("neg_div_double", scal.DivExpr(A("x"), A("i"))),
+    ("in_int", scal.InExpr(A("i"), [K("7", Type.INT), K("0", Type.INT), K("-9", Type.INT)])),
+    ("in_date", scal.InExpr(A("d"), [K("19950315", Type.DATE), K("20000229", Type.DATE)])),
+    ("not_or", scal.NotExpr(scal.OrExpr(scal.SmallerExpr(A("j"), K("2", Type.INT)),
+                                        scal.EqualsExpr(A("c"), K("A", Type.CHAR))))),
+    ("and_not", scal.AndExpr(scal.LargerEqualExpr(A("b"), K("3", Type.INT64)),
+                             scal.NotExpr(scal.EqualsExpr(A("i"), A("j"))))),
 ]
 
 _CTYPE = {"int32": "int32_t", "uint32": "uint32_t", "int8": "int8_t", "int64": "int64_t", "float32": "float",
@@ -115,7 +123,10 @@ def _cExpr(e):
     if isinstance(e, scal.ConstExpr):
         return {Type.FLOAT: "((float)%s)" % e.token, Type.CHAR: "'%s'" % e.token}.get(e.type, "(%s)" % e.token)
     ops = {scal.MulExpr: "*", scal.AddExpr: "+", scal.SubExpr: "-", scal.DivExpr: "/", scal.SmallerEqualExpr: "<=",
-           scal.LargerExpr: ">", scal.EqualsExpr: "=="}
+           scal.LargerExpr: ">", scal.EqualsExpr: "==", scal.SmallerExpr: "<", scal.LargerEqualExpr: ">=",
+           scal.AndExpr: "&&", scal.OrExpr: "||"}
+    if isinstance(e, scal.NotExpr):
+        return "(!%s)" % _cExpr(e.child)
     if type(e) in ops:
         return "(%s %s %s)" % (_cExpr(e.leftChild), ops[type(e)], _cExpr(e.rightChild))
     if isinstance(e, scal.ExtractExpr):
