@@ -15,8 +15,13 @@ Each candidate is timed with HIP events around every launch (median of a few
 runs), so one pipeline's knob is judged by that pipeline's kernel time; the
 table size factor is judged by the whole plan (it moves cost between the
 build, the probe and the init kernel).  Coordinate descent, one knob at a
-time, in a fixed order.  Results are cached as JSON per plan, variant and
-input-size bucket.
+time, in a fixed order.  Results are cached as JSON per plan and input-size
+bucket (log2 of the largest input table's rows); a record holds the file
+name of the module it was tuned on, whose name carries a hash of the
+generated plan source, the compiler flags and the device headers, so a
+record whose plan has changed since is ignored (its per-pipeline knobs could
+land on other kernels).  ``nearest`` picks the valid record of the bucket
+closest to a query's own size.
 """
 import json
 import math
@@ -150,41 +155,94 @@ class Autotuner(object):
                 trySet("ht_factor", ht["index"], [0.5, 0.75, 1.5, 2.0, 4.0], None)
         return best
 
-    def run(self):
-        """Tune every compile-time variant; returns the best record."""
+    def run(self, store=None, rows=None):
+        """Tune every compile-time variant; returns the best record.  With
+        `store` (a pre-filled column store, e.g. resident tables) the plan
+        reads the first rows[t] rows of each table from it."""
         results = []
-        for block, rows in self.variants:
-            comp = translateVariant(self.name, block, rows)
-            path = comp.compile(variantName(self.name, block, rows))
+        for block, rpt in self.variants:
+            path = variantPath(self.name, block, rpt)
             module = rt.PlanModule(path)
-            inputs = rt.BoundInputs(module, self.acc)
+            inputs = rt.BoundInputs(module, self.acc, store=store, rows=rows)
             tuning = rt.Tuning()
             times = self.tuneModule(module, inputs, tuning)
-            rec = {"plan": self.name, "block": block, "rows_per_thread": rows, "module": os.path.basename(path),
+            tabRows = {t: int(n) for t, n in zip(module.desc["tables"], inputs.rows)}
+            rec = {"plan": self.name, "block": block, "rows_per_thread": rpt, "module": os.path.basename(path),
+                   "bucket": bucketOf(max(tabRows.values()) if tabRows else 1), "rows": tabRows,
                    "tuning": tuningToDict(tuning), "kernels_ms": times, "total_ms": times["_total"]}
             if self.verbose:
-                print("variant b%d u%d: %.4f ms %s" % (block, rows, times["_total"],
+                print("variant b%d u%d: %.4f ms %s" % (block, rpt, times["_total"],
                                                      {k: round(v, 4) for k, v in times.items()}), flush=True)
             results.append(rec)
         results.sort(key=lambda r: r["total_ms"])
         return results[0], results
 
 
+def bucketOf(rows):
+    return int(math.log2(max(1, int(rows))))
+
+
 def sizeBucket(acc):
-    n = max(t["size"] for t in acc.schema.values()) if acc.schema else 1
-    return int(math.log2(max(1, n)))
+    return bucketOf(max(t["size"] for t in acc.schema.values()) if acc.schema else 1)
+
+
+def variantPath(name, block, rows):
+    """The compiled module of a variant (built ahead of time for the tuned
+    plans; compiled here otherwise)."""
+    return translateVariant(name, block, rows).compile(variantName(name, block, rows))
+
+
+def _recordPath(name, bucket):
+    return os.path.join(CACHE_DIR, "%s_n2e%d.json" % (name, bucket))
 
 
 def cachePath(name, acc):
-    return os.path.join(CACHE_DIR, "%s_n2e%d.json" % (name, sizeBucket(acc)))
+    return _recordPath(name, sizeBucket(acc))
+
+
+def isCurrent(rec):
+    """True when the record was tuned on the module the current translator
+    builds for its variant."""
+    try:
+        return os.path.basename(variantPath(rec["plan"], rec["block"], rec["rows_per_thread"])) == rec["module"]
+    except Exception:
+        return False
+
+
+def _load(path):
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec if isCurrent(rec) else None
 
 
 def loadCached(name, acc):
-    p = cachePath(name, acc)
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f)
-    return None
+    """The current record of the input-size bucket of `acc`, or None."""
+    return _load(cachePath(name, acc))
+
+
+def records(name):
+    """Every current record of a plan, by bucket."""
+    out = {}
+    if os.path.isdir(CACHE_DIR):
+        prefix = name + "_n2e"
+        for f in sorted(os.listdir(CACHE_DIR)):
+            if f.startswith(prefix) and f.endswith(".json") and f[len(prefix):-5].isdigit():
+                rec = _load(os.path.join(CACHE_DIR, f))
+                if rec is not None:
+                    out[int(f[len(prefix):-5])] = rec
+    return out
+
+
+def nearest(recs, rows, within=2):
+    """The record whose bucket is closest to that of `rows` driving rows
+    (ties: the larger bucket), if within `within` buckets; else None."""
+    if not recs:
+        return None
+    b = bucketOf(rows)
+    best = min(recs, key=lambda k: (abs(k - b), -k))
+    return recs[best] if abs(best - b) <= within else None
 
 
 def saveCached(name, acc, rec):
@@ -193,16 +251,58 @@ def saveCached(name, acc, rec):
         json.dump(rec, f, indent=1, sort_keys=True)
 
 
+def saveRecord(rec, directory=None):
+    d = directory or CACHE_DIR
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, os.path.basename(_recordPath(rec["plan"], rec["bucket"])))
+    with open(path, "w") as f:
+        json.dump(rec, f, indent=1, sort_keys=True)
+    return path
+
+
+def tuneServing(plans, scales=(0.1, 1.0, 10.0), tableScale=10.0, reps=5, out=None, log=print):
+    """Tune plans on HBM-resident tables at `tableScale` x the base sizes
+    (generated on the device), once per row scale: a query at scale s reads
+    the first s x base rows of every table, as the serving workload's do."""
+    from dogqc_amd import datagen, serving, workloads
+    tables = datagen.ResidentTables(scale=tableScale)
+    acc = tables.schemaAccessor()
+    saved = []
+    for name in plans:
+        config = planlib.PLANS[name][0]
+        for sc in scales:
+            rows = serving.prefixRows(tables, config, sc)
+            tuner = Autotuner(name, acc, reps=reps, verbose=False)
+            best, allRecs = tuner.run(store=tables.store, rows=rows)
+            best["scale"] = sc
+            path = saveRecord(best, out)
+            saved.append(path)
+            log("%s x%g (bucket %d): b%d u%d %.4f ms (variants %s) -> %s" % (
+                name, sc, best["bucket"], best["block"], best["rows_per_thread"], best["total_ms"],
+                " ".join("b%du%d:%.4f" % (r["block"], r["rows_per_thread"], r["total_ms"]) for r in allRecs),
+                os.path.basename(path)))
+    return saved
+
+
 def main(argv=None):
     import argparse
     from dogqc_amd import workloads
     from dogqc_amd.dbio import MemoryAccessor
     ap = argparse.ArgumentParser(description="autotune a named plan on its synthetic workload")
-    ap.add_argument("plan")
+    ap.add_argument("plan", nargs="+")
     ap.add_argument("--rows-scale", type=float, default=1.0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", default=None, help="also write the record to this path")
+    ap.add_argument("--serving", action="store_true",
+                    help="tune at 0.1x, 1x and 10x on device-resident tables (records to --out-dir)")
+    ap.add_argument("--scales", default="0.1,1,10")
+    ap.add_argument("--out-dir", default=None)
     args = ap.parse_args(argv)
+    if args.serving:
+        tuneServing(args.plan, [float(x) for x in args.scales.split(",")], reps=args.reps, out=args.out_dir,
+                    log=lambda m: print(m, flush=True))
+        return
+    args.plan = args.plan[0]
     kind = planlib.PLANS[args.plan][0]
     rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[kind].items()}
     acc = MemoryAccessor(workloads.generate(kind, rows))

@@ -95,30 +95,43 @@ class Server(object):
 
 
 class PlanSet(object):
-    """The plan module and runtime tuning of every config (the autotuned
-    variant when a record for the base size exists)."""
+    """The plan modules and runtime tunings of every config.  With
+    `tuned`, each current autotuner record of a config's plan (one per input
+    size bucket) gives a module variant and tuning, and a query takes the
+    record of the bucket nearest its driving rows (``pick``); queries with no
+    record within two buckets run the untuned plan.  ``modules`` /
+    ``tunings`` hold the base-size choice, used where one module per config
+    is needed (result decoding: every variant has the same result layout)."""
 
     def __init__(self, configs=tuple(CONFIG_PLANS), tuned=True):
         from dogqc_amd import autotune
         self.modules, self.tunings, self.names, self.batchModules = {}, {}, {}, {}
+        self.variants = {}
         for cfg in configs:
             name = CONFIG_PLANS[cfg]
-            rec = None
+            plain = (rt.PlanModule(planlib.translate(name).compile(name)), rt.newTuning())
+            byBucket = {}
             if tuned:
-                acc = workloads.emptyAccessor(cfg)
-                for t, n in workloads.FULL_ROWS[cfg].items():
-                    acc.schema[t]["size"] = n
-                rec = autotune.loadCached(name, acc)
-            if rec is not None:
-                comp = autotune.translateVariant(name, rec["block"], rec["rows_per_thread"])
-                mod = rt.PlanModule(comp.compile(autotune.variantName(name, rec["block"], rec["rows_per_thread"])))
-                tun = autotune.tuningFromDict(rec["tuning"])
-            else:
-                mod = rt.PlanModule(planlib.translate(name).compile(name))
-                tun = rt.newTuning()
-            self.modules[cfg], self.tunings[cfg], self.names[cfg] = mod, tun, name
+                for bucket, rec in autotune.records(name).items():
+                    path = autotune.variantPath(name, rec["block"], rec["rows_per_thread"])
+                    byBucket[bucket] = (rt.PlanModule(path), autotune.tuningFromDict(rec["tuning"]), rec)
+            self.variants[cfg] = (plain, byBucket)
+            base = max(workloads.FULL_ROWS[cfg].values())
+            self.modules[cfg], self.tunings[cfg] = self.pick(cfg, base)
+            self.names[cfg] = name
             bname = "%s_batch" % cfg
             self.batchModules[cfg] = rt.PlanModule(planlib.translate(bname).compile(bname))
+
+    def pick(self, cfg, drivingRows):
+        """(module, tuning) for a query of `cfg` whose largest table
+        prefix has `drivingRows` rows."""
+        from dogqc_amd import autotune
+        plain, byBucket = self.variants[cfg]
+        got = autotune.nearest({b: v for b, v in byBucket.items()}, drivingRows)
+        return (got[0], got[1]) if got is not None else plain
+
+    def tunedBuckets(self, cfg):
+        return sorted(self.variants[cfg][1])
 
 
 class Query(object):
@@ -170,11 +183,12 @@ class BatchBinding(object):
 
 def workspaceBytes(tables, plans, config, scale):
     """Workspace bytes of one query (its admission footprint)."""
-    r, cols = binding(plans.modules[config], tables, prefixRows(tables, config, scale))
+    rows = prefixRows(tables, config, scale)
+    mod, tun = plans.pick(config, max(rows.values()))
+    r, cols = binding(mod, tables, rows)
     inputs = rt.Inputs(len(r), len(cols), r.ctypes.data_as(C.POINTER(C.c_int64)), C.cast(cols, C.POINTER(C.c_void_p)))
     lay = rt.Layout()
-    rt.check(rt.lib().dqc_query_layout(plans.modules[config].handle, C.byref(inputs), C.byref(plans.tunings[config]),
-                                       C.byref(lay)), "dqc_query_layout")
+    rt.check(rt.lib().dqc_query_layout(mod.handle, C.byref(inputs), C.byref(tun), C.byref(lay)), "dqc_query_layout")
     return int(lay.ws_bytes)
 
 
@@ -211,14 +225,14 @@ def run(tables, plans, queries, slots, arena_bytes, keep=(), timeout_s=600, log=
     bindings = {}
     for i, q in enumerate(queries):
         rows = prefixRows(tables, q.config, q.scale)
-        r, cols = binding(plans.modules[q.config], tables, rows)
+        mod, tun = plans.pick(q.config, max(rows.values()))
+        r, cols = binding(mod, tables, rows)
         batch = None
         if max_batch > 1:
             if q.config not in bindings:
                 bindings[q.config] = BatchBinding(plans.modules[q.config], plans.batchModules[q.config], tables)
             batch = (plans.batchModules[q.config], bindings[q.config])
-        ids.append(srv.submit(plans.modules[q.config], r, cols, plans.tunings[q.config], q.arrival_ns,
-                              keep=i in keep, batch=batch))
+        ids.append(srv.submit(mod, r, cols, tun, q.arrival_ns, keep=i in keep, batch=batch))
     srv.start()
     deadline = time.perf_counter() + timeout_s
     while True:

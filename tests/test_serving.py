@@ -122,7 +122,9 @@ def test_admission_respects_arena_and_rejects_oversized():
     plans = serving.PlanSet()
     queries = [serving.Query("c1", 0.02) for _ in range(12)]
     need = serving.workspaceBytes(T, plans, "c1", 0.02)
-    r, cols = serving.binding(plans.modules["c1"], T, serving.prefixRows(T, "c1", 0.02))
+    rows = serving.prefixRows(T, "c1", 0.02)
+    mod, tun = plans.pick("c1", max(rows.values()))
+    r, cols = serving.binding(mod, T, rows)
     arena = int(need * 2.5)     # room for two at a time
     recs, srv = serving.run(T, plans, queries, 8, arena)
     events = sorted([(rc.admit_ns, 1) for rc in recs] + [(rc.done_ns, -1) for rc in recs])
@@ -133,7 +135,7 @@ def test_admission_respects_arena_and_rejects_oversized():
     assert peak <= 2
     srv.destroy()
     small = serving.Server(4, need // 2)
-    qid = small.submit(plans.modules["c1"], r, cols, plans.tunings["c1"])
+    qid = small.submit(mod, r, cols, tun)
     small.start()
     small.wait(60000)
     rec = small.record(qid)
@@ -144,12 +146,11 @@ def test_admission_respects_arena_and_rejects_oversized():
 def test_overflow_is_readmitted():
     print("gpu case: serving retry", flush=True)
     T = tables(0.02)
-    plans = serving.PlanSet()
+    plans = serving.PlanSet(tuned=False)
     t = rt.newTuning()
-    C.memmove(C.addressof(t), C.addressof(plans.tunings["c1"]), C.sizeof(t))
     t.ht_capacity[0] = 16
     t.ht_capacity[1] = 8
-    plans.tunings["c1"] = t
+    plans.variants["c1"] = ((plans.modules["c1"], t), {})
     q = serving.Query("c1", 0.02)
     recs, srv = serving.run(T, plans, [q], 2, 1 << 30, keep={0})
     assert recs[0].retries > 0
