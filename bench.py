@@ -84,35 +84,45 @@ def parse():
 
 
 def cpuBaseline(workload, g, seconds):
-    """The C oracle (oracle/cpu_oracle.c) on all host threads, repeated on the
-    full workload for about `seconds`; median time -> rows/s."""
+    """The C oracle (oracle/cpu_oracle.c) on the full workload: repeated for
+    about `seconds` on the host threads (at most 16, one GPU's share of a
+    GPU box's cores), then for about `seconds` / 2 on one thread; median
+    time -> rows/s."""
     from oracle import coracle
     from dogqc_amd import workloads
-    threads = os.cpu_count() or 1
     cols = {t: {k: v[1] for k, v in g[t].items()} for t in g}
     if workload.startswith("c1"):
-        run = lambda: coracle.c1(cols["dim"], cols["fact"], threads=threads)
+        fn = lambda th: coracle.c1(cols["dim"], cols["fact"], threads=th)
         rows = len(cols["fact"]["f_fk"])
     elif workload.startswith("c2"):
-        run = lambda: coracle.c2(cols["build"], cols["probe"], threads=threads)
+        fn = lambda th: coracle.c2(cols["build"], cols["probe"], threads=th)
         rows = len(cols["probe"]["p_key"])
     elif workload.startswith("c3"):
-        run = lambda: coracle.c3(cols, workloads.C3_WORD, threads=threads)
+        fn = lambda th: coracle.c3(cols, workloads.C3_WORD, threads=th)
         rows = len(cols["sales"]["s_item"])
     else:
-        run = lambda: coracle.c0(cols["cfg0"], threads=threads)
+        fn = lambda th: coracle.c0(cols["cfg0"], threads=th)
         rows = len(cols["cfg0"]["qty"])
-    run()
-    times = []
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        run()
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
+
+    def timed(threads, budget):
+        fn(threads)
+        times = []
+        t_end = time.perf_counter() + budget
+        while time.perf_counter() < t_end or len(times) < 3:
+            t0 = time.perf_counter()
+            fn(threads)
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times), times
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    med, times = timed(threads, seconds)
+    med1, times1 = timed(1, seconds / 2.0)
     return {"value": rows / med, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": "full workload (%d probe rows), %d repetitions in %.1f s, median %.2f ms"
-                      % (rows, len(times), sum(times), med * 1e3)}
+                      % (rows, len(times), sum(times), med * 1e3),
+            "single_thread": {"value": rows / med1, "unit": "rows/s", "cores": 1,
+                              "sample": "same workload, %d repetitions in %.1f s, median %.2f ms"
+                                        % (len(times1), sum(times1), med1 * 1e3)}}
 
 
 def main():
@@ -213,7 +223,11 @@ def main():
         "config": {"workload": spec["desc"], "rows": rows, "seed": workloads.SEEDS[kind],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": ("profiles/%s_%s_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                        "of this kernel, copy-kernel calibrated; read from the committed file, "
+                                        "not measured in this run" % (dom, args.workload)) if traffic else None,
+                     "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
         "kernels_ms": kernel_ms,
         "graph_ms_per_step": graph_ms,

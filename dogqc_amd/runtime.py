@@ -181,6 +181,18 @@ class DeviceBuffer(object):
         b.dtype, b.count = arr.dtype, len(arr)
         return b
 
+    @staticmethod
+    def fromMmdb(path):
+        """The payload of an mmdb column file, loaded by the runtime's C
+        reader (dqc_mmdb_load) into a new device allocation."""
+        device()
+        b = DeviceBuffer.__new__(DeviceBuffer)
+        b.ptr = C.c_void_p()
+        n = C.c_uint64()
+        check(lib().dqc_mmdb_load(path.encode(), C.byref(b.ptr), C.byref(n)), "dqc_mmdb_load")
+        b.nbytes = int(n.value)
+        return b
+
     def toArray(self, dtype, count, offset=0):
         out = np.empty(count, dtype=dtype)
         if out.nbytes:
@@ -289,6 +301,11 @@ class BoundInputs(object):
         self.ptrs = (C.c_void_p * max(1, len(desc["columns"])))()
         for c in desc["columns"]:
             key = (c["table"], c["attr"], c["part"])
+            if key not in self.store and hasattr(accessor, "file"):
+                # an mmdb directory (dbio.Accessor): the C reader loads the file
+                path = accessor.file(c["table"], c["attr"], c["part"])
+                if os.path.exists(path):
+                    self.store[key] = DeviceBuffer.fromMmdb(path)
             if key not in self.store:
                 data = accessor.column(c["table"], c["attr"])
                 if c["part"] == "offset":

@@ -661,6 +661,38 @@ def _money(rng, n, lo, hi):
     return (rng.integers(int(round(lo * 100)), int(round(hi * 100)) + 1, size=n) / 100.0).astype(np.float32)
 
 
+def _tblField(typ, data):
+    """One column as dbgen prints it: DATE yyyy-mm-dd, money with 2
+    decimals, CHAR one character, text verbatim."""
+    if typ == Type.STRING:
+        if isinstance(data, tuple):
+            offs, chars = data
+            raw = np.asarray(chars, dtype=np.uint8).tobytes()
+            return [raw[int(offs[i]):int(offs[i + 1])].decode("latin-1") for i in range(len(offs) - 1)]
+        return [x.decode("latin-1") for x in data]
+    a = np.asarray(data)
+    if typ == Type.DATE:
+        return ["%04d-%02d-%02d" % (d // 10000, d // 100 % 100, d % 100) for d in a.astype(np.int64)]
+    if typ in (Type.FLOAT, Type.DOUBLE):
+        return ["%.2f" % float(v) for v in a]
+    if typ == Type.CHAR:
+        return [chr(int(v) & 0xFF) for v in a]
+    return [str(int(v)) for v in a]
+
+
+def writeTbl(tables, directory):
+    """Write tables (as returned by ``generate``) as dbgen-format .tbl files
+    (one line per row, every field followed by '|'), the input of the
+    importer (dbio.dbAccess, dbio.py:117-253)."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    for name, cols in tables.items():
+        fields = [_tblField(typ, data) for typ, data in cols.values()]
+        with open(os.path.join(directory, name + ".tbl"), "w", encoding="latin-1", newline="\n") as f:
+            for row in zip(*fields):
+                f.write("|".join(row) + "|\n")
+
+
 def generate(sf=0.01, seed=1):
     """The eight TPC-H tables at scale factor `sf`, as plan-API column dicts
     (table -> OrderedDict attr -> (Type, data)); see dbio.MemoryAccessor."""

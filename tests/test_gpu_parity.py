@@ -323,6 +323,60 @@ def test_tpch(q, sf):
 
 
 
+_IMPORTED = {}
+
+
+def importedTpch(root):
+    """Synthetic TPC-H (12k lineitem rows) written as dbgen .tbl files and
+    imported through dbio.dbAccess (libdqcimport.so) into an mmdb directory;
+    plans read the column files through the runtime's C loader."""
+    if "acc" not in _IMPORTED:
+        from dogqc_amd import tpch, dbio
+        tpch.writeTbl(tpch.generate(0.002, seed=8), str(root / "tbl"))
+        _IMPORTED["acc"] = dbio.dbAccess(tpch.schema(), str(root / "mmdb"), str(root / "tbl"))
+    return _IMPORTED["acc"]
+
+
+@pytest.mark.parametrize("q", range(1, 23))
+def test_tpch_on_imported_tbl(q, tmp_path_factory):
+    """The reference's data path end to end (dbio.py:13-27, 117-253):
+    .tbl -> importer -> mmdb column files -> HBM (dqc_mmdb_load) -> HIP
+    plan, against the oracle over the same mmdb columns."""
+    from dogqc_amd import dbio
+    acc = importedTpch(tmp_path_factory.getbasetemp())
+    assert isinstance(acc, dbio.Accessor) and acc.schema["lineitem"]["size"] > 10000
+    check("tpch%d" % q, acc, storeKey=("imported",), label="imported .tbl")
+
+
+def test_plan_file_flow_on_gpu(tmp_path, monkeypatch, tmp_path_factory):
+    """A plan file and cfg file in the reference's format
+    (tests/golden/plans), evaluated as query/tpch.py:46-58 does, compiled and
+    executed through HipCompiler on imported data."""
+    import os
+    import dogqc_amd.scalarAlgebra as scal
+    from dogqc_amd.relationalAlgebra import RelationalAlgebra, Reduction
+    from dogqc_amd.types import Type
+    from dogqc_amd.util import loadScript
+    from dogqc_amd.hipTranslator import HipCompiler
+    from dogqc_amd.hiplang import CType
+    print("gpu case: plan file tpch10 on imported data", flush=True)
+    acc = importedTpch(tmp_path_factory.getbasetemp())
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plans")
+    monkeypatch.chdir(tmp_path)
+    alg = RelationalAlgebra(acc)
+    scope = {"alg": alg, "scal": scal, "Reduction": Reduction, "Type": Type}
+    plan = eval(loadScript(os.path.join(here, "tpch10.py")), scope)
+    cfg = eval(loadScript(os.path.join(here, "tpch10cfg.py")), scope)
+    plan = alg.resolveAlgebraPlan(plan, cfg)
+    comp = HipCompiler(algebraContext=alg, smArchitecture="sm_75", decimalRepr=CType.FP64, debug=False)
+    comp.gencode(alg.translateToCompilerPlan(plan, comp))
+    comp.compile("tpch10")
+    out = comp.execute()
+    want, rplan = oracleFor("tpch10", acc)
+    assertSameResult(comp.result, want, toleranceColumns(rplan), label="plan file tpch10")
+    assert "Result: %d tuples" % len(want["c_custkey"]) in out
+
+
 def test_reference_driver_flow_with_profiler(tmp_path, monkeypatch):
     """The reference's driver sequence (query/tpch.py:41-69, profiler.py:
     27-35): resolve, HipCompiler with profilers and a lane-refill buffer,

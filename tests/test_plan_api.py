@@ -79,6 +79,47 @@ def test_q10_generated_table_sizes():
     assert "dqc_spill_insert" not in desc["steps"]
 
 
+def planFileResolved():
+    """The reference's driver flow over a plan file and its cfg file
+    (query/tpch.py:46-58): eval(loadScript(path)) with alg, scal,
+    Reduction and Type in scope, then resolveAlgebraPlan."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plans")
+    acc = tpch.SchemaAccessor(tpch.schema())
+    alg = RelationalAlgebra(acc)
+    scope = {"alg": alg, "scal": scal, "Reduction": Reduction, "Type": Type}
+    plan = eval(loadScript(os.path.join(here, "tpch10.py")), scope)
+    cfg = eval(loadScript(os.path.join(here, "tpch10cfg.py")), scope)
+    return alg, alg.resolveAlgebraPlan(plan, cfg)
+
+
+def test_plan_file_flow_hits_q10_fixture():
+    """A plan file and a cfg file in the reference's format evaluate to the
+    Q10 plan of the reference's sample run: opIds 1-12, its edge estimates,
+    and the generated module's table sizes (sample/tpch10.cu: jht6 = 50,
+    jht5 = 150000 + 150000, jht9 = 300000, aht11 = 138748, result 69374)."""
+    alg, plan = planFileResolved()
+    byId = nodes(plan[-1])
+    assert sorted(byId) == list(range(1, 13))
+    est = {i: n.tupleNum for i, n in byId.items()}
+    assert (est[1], est[2], est[3], est[5], est[6], est[8], est[9], est[12]) == (
+        25, 1500000, 75000, 150000, 150000, 2040413, 69374, 69374)
+    comp = HipCompiler(algebraContext=alg, smArchitecture="sm_75", decimalRepr=CType.FP64, debug=False)
+    comp.gencode(alg.translateToCompilerPlan(plan, comp))
+    lib = ctypes.CDLL(comp.compile("tpch10"))
+    lib.dqc_plan_describe.restype = ctypes.c_char_p
+    desc = json.loads(lib.dqc_plan_describe().decode())
+    rows = np.array([tpch.SF1_ROWS[t] for t in desc["tables"]], dtype=np.int64)
+    cols = (ctypes.c_void_p * len(desc["columns"]))()
+    inp = rt.Inputs(len(rows), len(desc["columns"]), rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                    ctypes.cast(cols, ctypes.POINTER(ctypes.c_void_p)))
+    lay = rt.Layout()
+    assert lib.dqc_plan_layout_of(ctypes.byref(inp), ctypes.byref(rt.Tuning()), ctypes.byref(lay)) == 0
+    sizes = {h["label"]: (lay.ht_capacity[h["index"]], lay.mm_entries[h["index"]]) for h in desc["hts"]}
+    assert (sizes["jht6"], sizes["jht5"], sizes["jht9"], sizes["aht11"][0], lay.result_capacity) == (
+        (50, 0), (150000, 150000), (300000, 0), 138748, 69374)
+
+
 def test_opids_follow_construction_order():
     acc = workloads.emptyAccessor("c0")
     alg, p, cfg = workloads.plan("c0", acc)
