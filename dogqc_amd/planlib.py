@@ -410,6 +410,10 @@ for _c in ("c0", "c1", "c2", "c3"):
 # Q3 with a lane-refill buffer after the customer selection and a
 # lane-activity profile after the lineitem selection (profiler.py usage)
 PLANS["tpch3_profiled"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7]})
+# TPC-H Q10 as the reference's sample profiler reports ran it: a lane-activity
+# profile after the lineitem join (opId 9), without and with lane refill there
+PLANS["tpch10_p9"] = ("tpch", {"query": 10, "profilers": [9]})
+PLANS["tpch10_lr9_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9]})
 
 
 def build(name, acc):

@@ -349,12 +349,16 @@ def sameResult(config, a, b, rtol=1e-9):
 
 
 def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=4, verify=24, log=None,
-              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=8, batch_min_share=2.0, check=None):
+              max_heavy=1, heavy_mb=64.0, max_batch=16, batch_ratio=8, batch_min_share=2.0, check=None,
+              rate_queries=1000, high_load=3.0, high_queries=2000):
     """Serving benchmark on resident tables at `scale` x the base sizes.
-    Returns a dict of capacity (serial / batched QPS), latency at a fixed
-    Poisson rate (load x the serial capacity) and verification results.
-    `check(tables, config, scale, result) -> bool`, when given, is called
-    after the timed runs on each result of the verified sample."""
+    Returns a dict of capacity (serial / batched QPS, all queries waiting at
+    once), latency of both under a Poisson stream of `rate_queries` queries
+    at `load` x the serial capacity, latency of the batched loop under a
+    Poisson stream of `high_queries` at `high_load` x the serial capacity
+    (a rate the one-at-a-time loop cannot sustain), and verification
+    results.  `check(tables, config, scale, result) -> bool`, when given,
+    is called after the timed runs on each result of the verified sample."""
     from dogqc_amd import datagen
     say = log or (lambda *a: None)
     t0 = time.perf_counter()
@@ -372,7 +376,8 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     sample = []
     for c in CONFIG_PLANS:
         sample += [i for i in order if queries[i].config == c][:max(1, verify // len(CONFIG_PLANS))]
-    out = {"scale": scale, "resident_gb": tables.nbytes() / 1e9, "queries": nqueries, "slots": slots,
+    out = {"tuned_buckets": {c: plans.tunedBuckets(c) for c in CONFIG_PLANS},
+           "scale": scale, "resident_gb": tables.nbytes() / 1e9, "queries": nqueries, "slots": slots,
            "arena_gb": arena_gb, "max_heavy": max_heavy, "heavy_mb": heavy_mb, "max_batch": max_batch,
            "batch_ratio": batch_ratio, "batch_min_share": batch_min_share, "plans": dict(plans.names)}
     # capacity: every query arrives at t = 0
@@ -424,8 +429,9 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     srvB.destroy()
     say("verified %d queries: alone == mixed %d, checked %s" % (len(sample), alone_same,
                                                                  checked_ok if check is not None else "-"))
-    # latency at a fixed offered rate
+    # latency at a fixed offered rate, on a longer stream of the same mix
     rate = load * out["serial"]["qps"]
+    queries = makeMix(rate_queries, seed + 2)
     poissonArrivals(queries, rate, seed + 1)
     recs, srv = run(tables, plans, queries, 1, arena, log=say)
     srv.destroy()
@@ -442,6 +448,27 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     out["batched_at_rate"]["mean_batch"] = float(np.mean(sizes)) if sizes else 1.0
     out["offered_qps"] = rate
     out["speedup"] = out["batched"]["qps"] / out["serial"]["qps"]
+    # the batched loop at a rate the one-at-a-time loop cannot sustain
+    high = high_load * out["serial"]["qps"]
+    queries = makeMix(high_queries, seed + 4)
+    poissonArrivals(queries, high, seed + 3)
+    recs, srv = run(tables, plans, queries, slots, arena, log=say, **hv)
+    srv.destroy()
+    say("slowest at %.1f q/s: %s" % (high, slowest(queries, recs)))
+    st = latencyStats(recs)
+    sizes = [r.batch_size for r in recs if r.batch_size > 1]
+    st["in_batches"] = len(sizes)
+    st["mean_batch"] = float(np.mean(sizes)) if sizes else 1.0
+    st["offered_qps"] = high
+    st["offered_x_serial_capacity"] = high_load
+    # the stream kept up when completions span about the arrivals' span
+    span = (max(q.arrival_ns for q in queries) - min(q.arrival_ns for q in queries)) * 1e-6
+    st["arrival_span_ms"] = span
+    out["batched_at_high_rate"] = st
+    out["batched_at_high_rate_by_config"] = latencyByConfig(queries, recs)
+    say("at %.1f q/s (%.1fx the serial capacity): batched p50 %.2f ms, p99 %.2f ms, %d of %d in batches; "
+        "serial p99 at %.1f q/s: %.2f ms" % (high, high_load, st["p50_ms"], st["p99_ms"], st["in_batches"],
+                                             len(recs), rate, out["serial_at_rate"]["p99_ms"]))
     say("at %.1f q/s: p99 serial %.2f ms, batched %.2f ms" % (rate, out["serial_at_rate"]["p99_ms"],
                                                             out["batched_at_rate"]["p99_ms"]))
     return out
