@@ -16,13 +16,16 @@ runs), so one pipeline's knob is judged by that pipeline's kernel time; the
 table size factor is judged by the whole plan (it moves cost between the
 build, the probe and the init kernel).  Coordinate descent, one knob at a
 time, in a fixed order.  Results are cached as JSON per plan and input-size
-bucket (log2 of the largest input table's rows); a record holds the file
-name of the module it was tuned on, whose name carries a hash of the
-generated plan source, the compiler flags and the device headers, so a
-record whose plan has changed since is ignored (its per-pipeline knobs could
-land on other kernels).  ``nearest`` picks the valid record of the bucket
-closest to a query's own size.
+bucket (log2 of the largest input table's rows); a record holds a hash of
+the generated plan source of the variant it was tuned on, so a record whose
+plan has changed since is ignored (its per-pipeline grid, refill and table
+knobs are indexed by pipeline and table number and could land on other
+kernels).  The device headers and compiler flags are not part of that key:
+they change how fast a kernel runs, not which knob belongs to which kernel.
+``nearest`` picks the valid record of the bucket closest to a query's own
+size.
 """
+import hashlib
 import json
 import math
 import os
@@ -33,7 +36,7 @@ from dogqc_amd import planlib, runtime as rt
 PKG = os.path.dirname(os.path.abspath(__file__))
 CACHE_DIR = os.path.join(PKG, "_tuning")
 
-VARIANTS = [(256, 4), (256, 2), (256, 1), (512, 4), (128, 4), (256, 8)]
+VARIANTS = [(256, 4), (256, 2), (256, 1), (512, 4), (128, 4), (256, 8), (512, 1), (1024, 1)]
 
 
 def variantName(name, block, rows):
@@ -43,6 +46,11 @@ def variantName(name, block, rows):
 def translateVariant(name, block, rows, acc=None):
     comp = planlib.translate(name, acc, block=block, rowsPerThread=rows)
     return comp
+
+
+def planKey(source):
+    """Hash of a generated plan source: what a tuning record is valid for."""
+    return hashlib.sha256(source.encode()).hexdigest()[:20]
 
 
 def tuningToDict(t):
@@ -168,6 +176,7 @@ class Autotuner(object):
             times = self.tuneModule(module, inputs, tuning)
             tabRows = {t: int(n) for t, n in zip(module.desc["tables"], inputs.rows)}
             rec = {"plan": self.name, "block": block, "rows_per_thread": rpt, "module": os.path.basename(path),
+                   "plan_source": planKey(translateVariant(self.name, block, rpt).source),
                    "bucket": bucketOf(max(tabRows.values()) if tabRows else 1), "rows": tabRows,
                    "tuning": tuningToDict(tuning), "kernels_ms": times, "total_ms": times["_total"]}
             if self.verbose:
@@ -201,10 +210,11 @@ def cachePath(name, acc):
 
 
 def isCurrent(rec):
-    """True when the record was tuned on the module the current translator
-    builds for its variant."""
+    """True when the record was tuned on the plan source the current
+    translator generates for its variant."""
     try:
-        return os.path.basename(variantPath(rec["plan"], rec["block"], rec["rows_per_thread"])) == rec["module"]
+        return planKey(translateVariant(rec["plan"], rec["block"], rec["rows_per_thread"]).source) == \
+            rec["plan_source"]
     except Exception:
         return False
 
@@ -296,7 +306,7 @@ def main(argv=None):
     ap.add_argument("--serving", action="store_true",
                     help="tune at 0.1x, 1x and 10x on device-resident tables (records to --out-dir)")
     ap.add_argument("--scales", default="0.1,1,10")
-    ap.add_argument("--out-dir", default=None)
+    ap.add_argument("--out-dir", default=None, help="write records here instead of the package's record folder")
     args = ap.parse_args(argv)
     if args.serving:
         tuneServing(args.plan, [float(x) for x in args.scales.split(",")], reps=args.reps, out=args.out_dir,
@@ -308,7 +318,11 @@ def main(argv=None):
     acc = MemoryAccessor(workloads.generate(kind, rows))
     tuner = Autotuner(args.plan, acc, reps=args.reps, verbose=True)
     best, allRecs = tuner.run()
-    saveCached(args.plan, acc, best)
+    best["scale"] = args.rows_scale
+    if args.out_dir:
+        saveRecord(best, args.out_dir)
+    else:
+        saveCached(args.plan, acc, best)
     if args.out:
         with open(args.out, "w") as f:
             json.dump({"best": best, "all": allRecs}, f, indent=1, sort_keys=True)
