@@ -1,6 +1,7 @@
 """Ahead-of-time build: the runtime library and every named plan module.
 
-``buildAll()`` compiles ``csrc/dqc_runtime.hip`` and ``csrc/dqc_serve.hip``
+``buildAll()`` compiles ``csrc/dqc_runtime.hip``, ``csrc/dqc_serve.hip`` and
+``csrc/dqc_noalloc.hip``
 into ``libdqcrt.so``, ``csrc/dqc_gen.hip`` into ``libdqcgen.so``,
 ``csrc/dqc_import.cpp`` into ``libdqcimport.so`` (host code) and
 every plan of ``planlib.PLANS`` into ``_plans/`` (in parallel), always from
@@ -15,7 +16,7 @@ from concurrent.futures import ThreadPoolExecutor
 from dogqc_amd import plancache
 
 PKG = plancache.PKG
-RUNTIME_SRCS = [os.path.join(PKG, "csrc", f) for f in ("dqc_runtime.hip", "dqc_serve.hip")]
+RUNTIME_SRCS = [os.path.join(PKG, "csrc", f) for f in ("dqc_runtime.hip", "dqc_serve.hip", "dqc_noalloc.hip")]
 RUNTIME_SO = os.path.join(PKG, "libdqcrt.so")
 
 

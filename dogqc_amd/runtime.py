@@ -25,8 +25,9 @@ RUNTIME_SYMBOLS = [
     "dqc_module_unload", "dqc_module_describe", "dqc_query_layout", "dqc_query_enqueue", "dqc_query_run",
     "dqc_query_release", "dqc_graph_create", "dqc_graph_launch", "dqc_graph_destroy",
     "dqc_serve_create", "dqc_serve_submit", "dqc_serve_start", "dqc_serve_wait", "dqc_serve_record_of",
-    "dqc_serve_release", "dqc_serve_destroy",
+    "dqc_serve_release", "dqc_serve_destroy", "dqc_query_run_noalloc",
 ]
+NOALLOC_WS_BYTES = 64 << 20  # DQC_NOALLOC_WS_BYTES
 GEN_SYMBOLS = ["dqc_gen_uniform", "dqc_gen_decimal", "dqc_gen_pick_u32", "dqc_gen_affine_perm",
                "dqc_gen_inverse_cdf", "dqc_gen_text"]
 PLAN_SYMBOLS = ["dqc_plan_describe", "dqc_plan_layout_of", "dqc_plan_enqueue"]
@@ -130,6 +131,8 @@ def lib():
                                         C.POINTER(vp)]
         L.dqc_query_run.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), vp, C.c_int32, C.POINTER(QueryResult)]
         L.dqc_query_release.argtypes = [C.POINTER(QueryResult)]
+        L.dqc_query_run_noalloc.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), vp, C.c_int32,
+                                            C.POINTER(vp), C.c_int64, C.POINTER(QueryResult)]
         L.dqc_graph_create.argtypes = [vp, C.POINTER(Inputs), C.POINTER(Tuning), vp, C.c_uint64, vp, C.POINTER(vp)]
         L.dqc_graph_launch.argtypes = [vp, vp]
         L.dqc_graph_destroy.argtypes = [vp]
@@ -513,6 +516,20 @@ def run(module, inputs, tuning=None, stream=None, max_retries=8, feedback=None):
     if feedback is not None:
         feedback.learn(module, inputs, ex)
     return ex
+
+
+def runNoAlloc(module, inputs, outputs, out_rows, tuning=None, stream=None, max_retries=8):
+    """Run a plan in the runtime's static workspace (dqc_query_run_noalloc):
+    no device memory is allocated; the result rows are copied into
+    `outputs`, one caller-owned DeviceBuffer per result column holding
+    `out_rows` rows.  Returns the QueryResult (layout, status, retries)."""
+    tuning = tuning or Tuning()
+    stream = stream or Stream()
+    cols = (C.c_void_p * max(1, len(outputs)))(*[b.ptr.value for b in outputs])
+    res = QueryResult()
+    check(lib().dqc_query_run_noalloc(module.handle, C.byref(inputs.struct), C.byref(tuning), stream.handle,
+                                      max_retries, cols, out_rows, C.byref(res)), "dqc_query_run_noalloc")
+    return res
 
 
 def layout(module, inputs, tuning):
