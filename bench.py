@@ -52,6 +52,10 @@ for _v, _d in (("none", "per-lane match loops"), ("refill", "lane refill after t
         "table": "probe", "kernel": "krnl_probe2", "bytes_per_row": 4 + 4 + 8,
         "desc": "divergent expanding join (%s): build 1M Zipf(1.0) keys over 100k values, probe 16M, "
                 "filter p_f < 10, COUNT + SUM(int64 b_pay * p_pay)" % _d}
+# lane refill with the register buffer (one entry per lane, ds_permute /
+# ds_bpermute) in place of the LDS queue
+for _v in ("c1_refill", "c2_refill", "c2_both", "c3_refill"):
+    WORKLOADS[_v + "_reg"] = dict(WORKLOADS[_v], desc=WORKLOADS[_v]["desc"] + "; register refill buffer")
 
 
 SERVING = {"desc": "serving: stream of c0-c3 queries (row scale log-uniform 0.1x-10x) over HBM-resident tables "
@@ -249,11 +253,11 @@ def benchQ10(args):
     """TPC-H Q10 on synthetic data at --sf (SF1: 6,001,215 lineitem rows),
     the query of the reference's only published measurements (sample
     profiler reports: krnl_lineitem7 15.2 ms divergent, 5.1 ms with lane
-    refill at opId 9; RTX 2080, FP32 decimals, grid 100 x 32).  Both plans
+    refill at opId 9; RTX 2080, FP32 decimals, grid 100 x 32).  The refill
+    runs with the LDS queue and with the register buffer.  All plans
     carry the lane-activity profile after the lineitem join (opId 9);
     kernel times are taken with the profile off, the histogram in one more
-    run with it on.  Results of the two plans are checked against each
-    other and against the oracle."""
+    run with it on.  Every plan's result is checked against the oracle."""
     import numpy as np
     from dogqc_amd import planlib, runtime as rt, runner, tpch
     from dogqc_amd.dbio import MemoryAccessor
@@ -274,7 +278,7 @@ def benchQ10(args):
            "plans": {}}
     results = {}
     stream = rt.Stream()
-    for name in ("tpch10_p9", "tpch10_lr9_p9"):
+    for name in ("tpch10_p9", "tpch10_lr9_p9", "tpch10_lr9reg_p9"):
         mod = rt.PlanModule(planlib.translate(name).compile(name))
         inputs = rt.BoundInputs(mod, acc, store=store)
         tun = rt.newTuning()
@@ -308,14 +312,15 @@ def benchQ10(args):
                              "wave_iterations": iters, "mean_active_lanes_of_64": mean}}
         print("[q10] %s: lineitem7 %.3f ms, total %.3f ms, mean active lanes %.1f of 64 over %d wave trips"
               % (name, per["krnl_lineitem7"], sum(per.values()), mean, iters), file=sys.stderr, flush=True)
-    a, b = results["tpch10_p9"], results["tpch10_lr9_p9"]
+    a, b, c = results["tpch10_p9"], results["tpch10_lr9_p9"], results["tpch10_lr9reg_p9"]
     from oracle.planeval import evaluate
     alg, p, cfg, _ = planlib.build("tpch10", acc)
     plan = alg.resolveAlgebraPlan(p, cfg)
     want = evaluate(plan, acc)
     out["parity"] = {"oracle": "planeval", "groups": int(len(want["c_custkey"])),
                      "plain_ok": sameGroups(a, want, ["c_custkey"], ["revenue"]),
-                     "refill_ok": sameGroups(b, want, ["c_custkey"], ["revenue"])}
+                     "refill_ok": sameGroups(b, want, ["c_custkey"], ["revenue"]),
+                     "register_refill_ok": sameGroups(c, want, ["c_custkey"], ["revenue"])}
     best = min(out["plans"].values(), key=lambda r: r["total_ms"])
     out["value"] = li / (best["total_ms"] * 1e-3)
     out["ms_per_step"] = best["total_ms"]

@@ -70,6 +70,22 @@ DQC_DEVICE_INLINE T shfl(T v, int src) {
     return r;
 }
 
+// Forward permute: this lane's value goes to lane `dst`, and the value that
+// some lane sent to this one is returned (ds_permute_b32 per 32-bit word, no
+// LDS memory).  Every lane of the wavefront takes part, with distinct `dst`.
+template <typename T>
+DQC_DEVICE_INLINE T permute_to(T v, uint32_t dst) {
+    constexpr int N = (int)((sizeof(T) + 3) / 4);
+    uint32_t w[N];
+    for (int i = 0; i < N; ++i) w[i] = 0u;
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = (uint32_t)__builtin_amdgcn_ds_permute((int)(dst * 4u), (int)w[i]);
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
 // a value known to be equal in all lanes, moved to scalar registers
 DQC_DEVICE_INLINE uint64_t uniform64(uint64_t v) {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);

@@ -7,24 +7,48 @@ position run only once at least ``threshold`` x lanes of the (warp) wavefront
 can be filled; at the end of the input the buffer drains.  Positions inside
 inner loops are skipped (cudaDivergenceBuffer.py:21-24).
 
-gfx950 design: every wavefront owns a 128-entry LDS queue per live register
-(attributes already in registers, plus the row indices of attributes that are
-still loaded lazily, so columns that only later operators read are fetched
-after compaction, for dense lanes only).  Survivors are appended with one
-64-bit ballot and an mbcnt prefix; the downstream code runs on up to 64
-popped entries at a time.  The release threshold is a run-time kernel
-parameter (1..64 lanes), so the autotuner can change it without recompiling.
+Two buffer kinds, selected like the reference's BufferType
+(cudaDivergenceBuffer.py:44-46) through ``LaneRefill.bufferType``:
+
+* ``BufferType.SMEM`` (default): every wavefront owns a 128-entry LDS queue
+  per live register (attributes already in registers, plus the row indices
+  of attributes that are still loaded lazily, so columns that only later
+  operators read are fetched after compaction, for dense lanes only).
+  Survivors are appended with one 64-bit ballot and an mbcnt prefix; the
+  downstream code runs on up to 64 popped entries at a time.
+
+* ``BufferType.REG``: the buffer is one register per live register and lane,
+  64 entries per wavefront, no LDS (the reference's register variant,
+  cudaDivergenceBuffer.py:73-113, 129-164, 180-209, which its BufferedLoop
+  never selects).  When the active lanes plus the buffered entries reach the
+  threshold, inactive lanes are refilled from the top of the buffer with one
+  ds_bpermute per 32-bit word and the operators above run; otherwise the
+  active lanes are pushed onto the buffer with one ds_permute per word, each
+  lane sending to a distinct lane (active lanes to the buffer's free
+  entries, inactive lanes to the rest), so the permutation is total.
+
+The release threshold is a run-time kernel parameter (1..64 lanes), so the
+autotuner can change it without recompiling.
 """
+from enum import Enum
+
 from dogqc_amd.translatorBase import UnaryTranslator
+
+
+class BufferType(Enum):
+    SMEM = 1
+    REG = 2
 
 
 class LaneRefill(UnaryTranslator):
 
     threshold = 0.9
+    bufferType = BufferType.SMEM
 
     def __init__(self, parentAlgExpr, child):
         UnaryTranslator.__init__(self, parentAlgExpr, child)
         self.threshold = LaneRefill.threshold
+        self.bufferType = LaneRefill.bufferType
 
     def produce(self, ctxt):
         self.child.produce(ctxt)
@@ -44,12 +68,6 @@ class LaneRefill(UnaryTranslator):
         live = pipe.liveState(self.algExpr.outRelation.values())
         if pipe.tid not in live:
             live[pipe.tid] = "int64_t"
-        nw = pipe.bs // 64
-        queues = []
-        for r, ct in live.items():
-            q = "rq%d_%s" % (X, r)
-            pipe.staticLds.append("__shared__ %s %s[%d][128];" % (ct, q, nw))
-            queues.append((r, q))
         cnt = pipe.reg("rcnt%d" % X, "uint32_t", "0", perRow=False)
         default = max(1, min(64, int(64 * self.threshold)))
         pid = pipe.pipeId
@@ -57,6 +75,24 @@ class LaneRefill(UnaryTranslator):
                          "std::max<int32_t>(1, std::min<int32_t>(64, t->refill_threshold[%d] > 0 ? "
                          "t->refill_threshold[%d] : %d))" % (pid, pid, default))
         outer = pipe.refillPoints[-1] if pipe.refillPoints else None
+        if self.bufferType is BufferType.REG:
+            self._consumeReg(ctxt, pipe, X, live, cnt, thr, outer)
+        else:
+            self._consumeSmem(ctxt, pipe, X, live, cnt, thr, outer)
+
+    def _flushFlag(self, pipe, X, outer):
+        if outer is None:
+            pipe.emit("const bool rflush%d = !more;" % X)
+        else:
+            pipe.emit("const bool rflush%d = rflush%d && %s == 0u;" % (X, outer["id"], outer["cnt"]))
+
+    def _consumeSmem(self, ctxt, pipe, X, live, cnt, thr, outer):
+        nw = pipe.bs // 64
+        queues = []
+        for r, ct in live.items():
+            q = "rq%d_%s" % (X, r)
+            pipe.staticLds.append("__shared__ %s %s[%d][128];" % (ct, q, nw))
+            queues.append((r, q))
         pipe.emit("// lane refill (opId %d)" % X)
         pipe.code.open("")
         pipe.emit("const uint64_t rm = dqc::ballot(active[u]);")
@@ -70,10 +106,7 @@ class LaneRefill(UnaryTranslator):
         # LDS stores before the pops below
         pipe.emit("dqc::wave_lds_sync();")
         pipe.code.close()
-        if outer is None:
-            pipe.emit("const bool rflush%d = !more;" % X)
-        else:
-            pipe.emit("const bool rflush%d = rflush%d && %s == 0u;" % (X, outer["id"], outer["cnt"]))
+        self._flushFlag(pipe, X, outer)
         point = {"id": X, "cnt": cnt}
         point["mark"] = pipe.code.mark()
         pipe.code.depth += 1
@@ -89,6 +122,54 @@ class LaneRefill(UnaryTranslator):
         pipe.emit("dqc::wave_lds_sync();")
         point["cond"] = "%s >= (rflush%d ? 1u : %s)" % (cnt, X, thr)
         self.parent.consume(ctxt)
+        pipe.code.close()
+        pipe.emit("active[u] = 0;")
+
+    def _consumeReg(self, ctxt, pipe, X, live, cnt, thr, outer):
+        pipe.annotations.append("REG%d" % X)
+        bufs = []
+        for r, ct in live.items():
+            bufs.append((r, pipe.reg("rb%d_%s" % (X, r), ct, "{}", perRow=False)))
+        go = pipe.reg("rgo%d" % X, "bool", "false", perRow=False)
+        pipe.emit("// lane refill, register buffer (opId %d)" % X)
+        self._flushFlag(pipe, X, outer)
+        pipe.code.open("")
+        pipe.emit("const uint64_t rm = dqc::ballot(active[u]);")
+        pipe.emit("const uint32_t rna = dqc::popc64(rm);")
+        pipe.emit("%s = %s + rna >= %s || (rflush%d && %s + rna > 0u);" % (go, cnt, thr, X, cnt))
+        # below the threshold: park the active lanes at entries cnt.. of the
+        # buffer (cnt + rna < thr <= 64, so they fit)
+        pipe.code.open("if (!%s)" % go)
+        pipe.emit("const uint32_t lb = dqc::lanes_below(rm);")
+        pipe.emit("const uint32_t rdst = (%s + (active[u] ? lb : rna + (lane - lb))) & 63u;" % cnt)
+        pipe.emit("const bool rin = lane >= %s && lane < %s + rna;" % (cnt, cnt))
+        for r, b in bufs:
+            pipe.emit("{ const auto rv = dqc::permute_to(%s[u], rdst); if (rin) %s = rv; }" % (r, b))
+        pipe.emit("%s += rna;" % cnt)
+        pipe.emit("active[u] = 0;")
+        pipe.code.close()
+        pipe.code.close()
+        point = {"id": X, "cnt": cnt}
+        point["mark"] = pipe.code.mark()
+        pipe.code.depth += 1
+        pipe.refillPoints.append(point)
+        # fill the inactive lanes from the top of the buffer
+        pipe.code.open("")
+        pipe.emit("const uint64_t rm = dqc::ballot(active[u]);")
+        pipe.emit("const uint32_t rna = dqc::popc64(rm);")
+        pipe.emit("const uint32_t rtake = %s < 64u - rna ? %s : 64u - rna;" % (cnt, cnt))
+        pipe.emit("const uint32_t rs = lane - dqc::lanes_below(rm);")
+        pipe.emit("const bool rdest = !active[u] && rs < rtake;")
+        pipe.emit("const uint32_t rsrc = rdest ? %s - rtake + rs : lane;" % cnt)
+        for r, b in bufs:
+            pipe.emit("{ const auto rv = dqc::shfl(%s, (int)rsrc); if (rdest) %s[u] = rv; }" % (b, r))
+        pipe.emit("if (rdest) active[u] = 1;")
+        pipe.emit("%s -= rtake;" % cnt)
+        pipe.code.close()
+        point["cond"] = go
+        self.parent.consume(ctxt)
+        pipe.emit("active[u] = 0;")
+        pipe.emit("%s = %s >= %s || (rflush%d && %s > 0u);" % (go, cnt, thr, X, cnt))
         pipe.code.close()
         pipe.emit("active[u] = 0;")
 

@@ -19,16 +19,17 @@ from dogqc_amd.dbio import MemoryAccessor
 from dogqc_amd.hiplang import CType
 from dogqc_amd.hipTranslator import HipCompiler
 from dogqc_amd.hashJoins import EquiJoinTranslator
-from dogqc_amd.hipDivergenceBuffer import LaneRefill
+from dogqc_amd.hipDivergenceBuffer import LaneRefill, BufferType
 from dogqc_amd.kernel import KernelCall
 
 
 @contextlib.contextmanager
-def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBuild=False):
+def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBuild=False, bufferType=None):
     old = (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
-           KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild)
+           KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType)
     EquiJoinTranslator.usePushDownJoin = pushdown
     EquiJoinTranslator.usePartitionedBuild = partBuild
+    LaneRefill.bufferType = bufferType or BufferType.SMEM
     if block is not None:
         KernelCall.defaultBlockSize = block
     if threshold is not None:
@@ -39,7 +40,7 @@ def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBu
         yield
     finally:
         (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
-         KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild) = old
+         KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType) = old
 
 
 # ------------------------------------------------------------- feature plans
@@ -377,6 +378,8 @@ PLANS["c2_none"] = ("c2", {})
 PLANS["c2_refill"] = ("c2", {"buffers": [3]})
 PLANS["c2_pushdown"] = ("c2", {"pushdown": True})
 PLANS["c2_both"] = ("c2", {"buffers": [3], "pushdown": True})
+PLANS["c2_refill_reg"] = ("c2", {"buffers": [3], "buffer_type": "reg"})
+PLANS["c2_both_reg"] = ("c2", {"buffers": [3], "pushdown": True, "buffer_type": "reg"})
 PLANS["c1_refill"] = ("c1", {"buffers": [4]})
 PLANS["c3"] = ("c3", {})
 PLANS["c3_refill"] = ("c3", {"buffers": [6]})
@@ -387,6 +390,7 @@ for _n in STRING_PLANS:
     PLANS[_n] = ("strings", {})
 PLANS["f_join_multi_pushdown"] = ("feature", {"pushdown": True, "base": "f_join_multi"})
 PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_project"})
+PLANS["f_like_project_refill_reg"] = ("feature", {"buffers": [2], "base": "f_like_project", "buffer_type": "reg"})
 
 
 # TPC-H Q1-Q22 (dogqc_amd/tpch.py)
@@ -414,6 +418,11 @@ PLANS["tpch3_profiled"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7]
 # profile after the lineitem join (opId 9), without and with lane refill there
 PLANS["tpch10_p9"] = ("tpch", {"query": 10, "profilers": [9]})
 PLANS["tpch10_lr9_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9]})
+# lane refill with the register buffer (BufferType.REG) in place of the LDS queue
+PLANS["tpch10_lr9reg_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9], "buffer_type": "reg"})
+PLANS["tpch3_profiled_reg"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7], "buffer_type": "reg"})
+PLANS["c1_refill_reg"] = ("c1", {"buffers": [4], "buffer_type": "reg"})
+PLANS["c3_refill_reg"] = ("c3", {"buffers": [6], "buffer_type": "reg"})
 
 
 def build(name, acc):
@@ -457,7 +466,8 @@ def emptyAccessor(name):
 
 
 # plans the autotuner explores compile-time variants of (built ahead of time)
-TUNE_PLANS = ["c0", "c1", "c1_refill", "c2_none", "c2_refill", "c2_pushdown", "c2_both", "c3", "c3_refill"]
+TUNE_PLANS = ["c0", "c1", "c1_refill", "c2_none", "c2_refill", "c2_pushdown", "c2_both", "c3", "c3_refill",
+              "c1_refill_reg", "c2_refill_reg", "c2_both_reg", "c3_refill_reg"]
 
 
 def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=None):
@@ -469,7 +479,8 @@ def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=
     plan = alg.resolveAlgebraPlan(p, cfg)
     with knobs(pushdown=opts.get("pushdown", False), block=block or opts.get("block"),
                threshold=opts.get("threshold"), rowsPerThread=rowsPerThread,
-               partBuild=opts.get("part_build", False)):
+               partBuild=opts.get("part_build", False),
+               bufferType=BufferType.REG if opts.get("buffer_type") == "reg" else BufferType.SMEM):
         comp = HipCompiler(alg, "gfx950", decimalRepr)
         comp.setBuffers(opts.get("buffers", []))
         comp.setProfilers(opts.get("profilers", []))

@@ -82,12 +82,12 @@ def test_c0_grid(grid):
     check("c0", workloadAcc("c0", {"cfg0": 300000}, seed=6), tuning=t, label="grid=%d" % grid)
 
 
-@pytest.mark.parametrize("plan", ["c1", "c1_refill"])
+@pytest.mark.parametrize("plan", ["c1", "c1_refill", "c1_refill_reg"])
 def test_c1_full(plan):
     check(plan, workloadAcc("c1"), storeKey="c1full", label="1.5M x 6M")
 
 
-@pytest.mark.parametrize("plan", ["c1", "c1_refill"])
+@pytest.mark.parametrize("plan", ["c1", "c1_refill", "c1_refill_reg"])
 @pytest.mark.parametrize("rows", [{"dim": 0, "fact": 1000}, {"dim": 1000, "fact": 0}, {"dim": 1, "fact": 1},
                                   {"dim": 3000, "fact": 20000}])
 def test_c1_small(rows, plan):
@@ -103,23 +103,27 @@ def test_c1_undersized_tables_retry():
     assert retries > 0
 
 
-@pytest.mark.parametrize("variant", ["c2_none", "c2_refill", "c2_pushdown", "c2_both"])
+C2_VARIANTS = ["c2_none", "c2_refill", "c2_pushdown", "c2_both", "c2_refill_reg", "c2_both_reg"]
+
+
+@pytest.mark.parametrize("variant", C2_VARIANTS)
 def test_c2_small(variant):
     check(variant, workloadAcc("c2", {"build": 2000, "probe": 50000}, seed=31), label="small")
 
 
-@pytest.mark.parametrize("variant", ["c2_none", "c2_refill", "c2_pushdown", "c2_both"])
+@pytest.mark.parametrize("variant", C2_VARIANTS)
 def test_c2_empty(variant):
     check(variant, workloadAcc("c2", {"build": 0, "probe": 100}, seed=32), label="empty build")
     check(variant, workloadAcc("c2", {"build": 100, "probe": 0}, seed=33), label="empty probe")
 
 
-@pytest.mark.parametrize("thr", [1, 17, 64])
-def test_c2_refill_thresholds(thr):
+@pytest.mark.parametrize("plan", ["c2_both", "c2_both_reg", "c2_refill_reg"])
+@pytest.mark.parametrize("thr", [1, 17, 63, 64])
+def test_c2_refill_thresholds(thr, plan):
     t = rt.newTuning()
     for i in range(rt.MAX_PIPES):
         t.refill_threshold[i] = thr
-    check("c2_both", workloadAcc("c2", {"build": 5000, "probe": 100000}, seed=34), tuning=t, label="thr=%d" % thr)
+    check(plan, workloadAcc("c2", {"build": 5000, "probe": 100000}, seed=34), tuning=t, label="thr=%d" % thr)
 
 
 @pytest.mark.parametrize("thr", [1, 7, 100000])
@@ -165,7 +169,7 @@ def c2ClosedForm(acc):
     return cnt, (s + 2 ** 63) % 2 ** 64 - 2 ** 63
 
 
-@pytest.mark.parametrize("variant", ["c2_none", "c2_refill", "c2_pushdown", "c2_both"])
+@pytest.mark.parametrize("variant", C2_VARIANTS)
 def test_c2_full_closed_form(variant):
     acc = _STORE.setdefault("c2acc", workloadAcc("c2"))
     print("gpu case: %s full 1M x 16M" % variant, flush=True)
@@ -201,7 +205,7 @@ def c3Oracle(acc):
     return res, ev.temps["c3_by_nation_year"][0]
 
 
-@pytest.mark.parametrize("plan", ["c3", "c3_refill"])
+@pytest.mark.parametrize("plan", ["c3", "c3_refill", "c3_refill_reg"])
 @pytest.mark.parametrize("rows", [C3_SMALL, {"sales": 1, "item": 1, "cust": 1, "nation": 1},
                                   {"sales": 0, "item": 100, "cust": 10, "nation": 25},
                                   {"sales": 5000, "item": 0, "cust": 10, "nation": 25},
@@ -228,12 +232,13 @@ def test_c3_temp_overflow_retry():
     assertSameResult(ny, wantNy, {"revenue"}, label="c3 retry by nation, year")
 
 
-def test_c3_full_vs_c_oracle():
+@pytest.mark.parametrize("plan", ["c3", "c3_refill_reg"])
+def test_c3_full_vs_c_oracle(plan):
     from oracle import coracle
-    print("gpu case: c3 full 60M x 2M x 200k x 25", flush=True)
+    print("gpu case: %s full 60M x 2M x 200k x 25" % plan, flush=True)
     data = workloads.generate("c3")
     acc = MemoryAccessor(data)
-    got, ny, _ = c3Run(acc)
+    got, ny, _ = c3Run(acc, plan=plan, storeKey="c3full")
     tables = {t: {a: v[1] for a, v in cols.items()} for t, cols in data.items()}
     wantItem, wantNy = coracle.c3(tables, workloads.C3_WORD)
     assert len(wantItem["i_key"]) > 90000 and len(wantNy["n_key"]) == 175
@@ -320,6 +325,15 @@ def test_tpch(q, sf):
     """TPC-H Q1-Q22 on synthetic TPC-H data (300k and 12k lineitem rows),
     every operator of the plan API on the GPU."""
     check("tpch%d" % q, tpchAcc(sf), storeKey=("tpch", sf), label="sf=%g" % sf)
+
+
+@pytest.mark.parametrize("name", ["tpch10_p9", "tpch10_lr9_p9", "tpch10_lr9reg_p9", "tpch3_profiled",
+                                  "tpch3_profiled_reg"])
+@pytest.mark.parametrize("sf", [0.05, 0.002])
+def test_tpch_refill_variants(name, sf):
+    """Q10 and Q3 with lane refill (LDS queue and register buffer) and
+    lane-activity profiles in the plan, against the oracle."""
+    check(name, tpchAcc(sf), storeKey=("tpch", sf), label="sf=%g" % sf)
 
 
 
