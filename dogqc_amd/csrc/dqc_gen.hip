@@ -49,6 +49,19 @@ __global__ void __launch_bounds__(BS) k_affine_perm(int32_t* out, int64_t n, uin
         out[i] = (int32_t)((a * (uint64_t)i + b) % (uint64_t)n);
 }
 
+// row r of a table of nkeys rows (keys from the affine permutation a, b),
+// r uniform over the first ceil(nkeys * (i + 1) / n) rows
+__global__ void __launch_bounds__(BS) k_prefix_key(int32_t* out, int64_t n, uint64_t seed, uint32_t column,
+                                                   int64_t nkeys, uint64_t a, uint64_t b) {
+    for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
+        const unsigned __int128 num = (unsigned __int128)(uint64_t)nkeys * (uint64_t)(i + 1);
+        uint64_t rows = (uint64_t)((num + (uint64_t)n - 1u) / (uint64_t)n);
+        rows = rows < 1u ? 1u : (rows > (uint64_t)nkeys ? (uint64_t)nkeys : rows);
+        const uint64_t r = (uint64_t)uniformOf(rnd(seed, column, (uint64_t)i), 0, rows);
+        out[i] = (int32_t)((a * r + b) % (uint64_t)nkeys);
+    }
+}
+
 __global__ void __launch_bounds__(BS) k_inverse_cdf(int32_t* out, int64_t n, uint64_t seed, uint32_t column,
                                                     const double* cdf, const int32_t* values, int32_t nvalues) {
     for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n; i += (int64_t)gridDim.x * BS) {
@@ -144,6 +157,17 @@ int dqc_gen_affine_perm(int32_t* out, int64_t n, uint64_t a, uint64_t b, void* s
     if (n < 0 || n >= (1ll << 31) || (n > 0 && (!out || a >= (uint64_t)n || b >= (uint64_t)n))) return DQC_EINVAL;
     if (n == 0) return DQC_OK;
     hipLaunchKernelGGL(k_affine_perm, dim3(gridFor(n)), dim3(BS), 0, (hipStream_t)stream, out, n, a, b);
+    return check();
+}
+
+int dqc_gen_prefix_key(int32_t* out, int64_t n, uint64_t seed, uint32_t column, int64_t nkeys, uint64_t a,
+                       uint64_t b, void* stream) {
+    if (n < 0 || nkeys <= 0 || nkeys >= (1ll << 31) || a >= (uint64_t)nkeys || b >= (uint64_t)nkeys ||
+        (n > 0 && !out))
+        return DQC_EINVAL;
+    if (n == 0) return DQC_OK;
+    hipLaunchKernelGGL(k_prefix_key, dim3(gridFor(n)), dim3(BS), 0, (hipStream_t)stream, out, n, seed, column, nkeys,
+                       a, b);
     return check();
 }
 

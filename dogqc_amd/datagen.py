@@ -6,8 +6,14 @@ columns) and runs every query on a row prefix of them.  Generating that
 much data on the host and copying it over would take minutes, so the
 columns are filled by device kernels, each value a pure function of (seed,
 column id, row) (include/dogqc_amd_datagen.h).  The distributions are those
-of ``workloads.generate``; the random streams differ (counter-based here,
-PCG64 there), so the two are different, equally valid, draws.
+of ``workloads.generate``, except for the foreign keys of c1 (f_fk) and c3
+(s_item, s_cust): a query reads a prefix of every table, so a key drawn
+uniformly over the whole referenced table would miss that table's prefix
+(at 1x of 10x tables, nine keys in ten).  Those keys are drawn so that every
+prefix of the referencing table stays inside the same fraction of the
+referenced one (dqc_gen_prefix_key), which keeps each config's join
+selectivities at every query scale.  The random streams differ
+(counter-based here, PCG64 there), so the two are different draws.
 
 ``ResidentTables.store`` maps (table, attr, part) to a device buffer in the
 form ``runtime.BoundInputs`` takes as a pre-filled column store, and
@@ -46,6 +52,7 @@ def lib():
         L.dqc_gen_affine_perm.argtypes = [vp, i64, u64, u64, vp]
         L.dqc_gen_inverse_cdf.argtypes = [vp, i64, u64, u32, vp, vp, i32, vp]
         L.dqc_gen_text.argtypes = [vp, vp, i64, i32, u64, u32, vp, vp, i32, i32, vp]
+        L.dqc_gen_prefix_key.argtypes = [vp, i64, u64, u32, i64, u64, u64, vp]
         _lib = L
     return _lib
 
@@ -120,6 +127,15 @@ class ResidentTables(object):
         b = self._new(table, attr, n * 4)
         _check(lib().dqc_gen_pick_u32(b.ptr, n, self.seed, self._colId(table, attr), tab.ptr, 2557, None), attr)
 
+    def _prefixKey(self, table, attr, ref):
+        """Foreign keys into table `ref` (keys from _perm) that a query
+        reading the first f x rows of both tables always finds: a query at
+        any scale keeps the config's join selectivities."""
+        n, nk = self.rows[table], self.rows[ref]
+        b = self._new(table, attr, n * 4)
+        a, c = permParams(nk)
+        _check(lib().dqc_gen_prefix_key(b.ptr, n, self.seed, self._colId(table, attr), nk, a, c, None), attr)
+
     def _perm(self, table, attr):
         n = self.rows[table]
         b = self._new(table, attr, n * 4)
@@ -141,7 +157,7 @@ class ResidentTables(object):
         self._perm("dim", "d_key")
         self._uniform("dim", "d_cat", I8, ord("0"), ord("4"))
         self._uniform("dim", "d_seg", I32, 0, 999999)
-        self._uniform("fact", "f_fk", I32, 0, self.rows["dim"] - 1)
+        self._prefixKey("fact", "f_fk", "dim")
         self._decimal("fact", "f_price", 90000, 10500000, 100)
         self._decimal("fact", "f_disc", 0, 10, 100)
         self._dates("fact", "f_date")
@@ -183,8 +199,8 @@ class ResidentTables(object):
         self._uniform("cust", "c_year", I32, 1992, 1998)
         self._perm("nation", "n_key")
         t = "sales"
-        self._uniform(t, "s_item", I32, 0, self.rows["item"] - 1)
-        self._uniform(t, "s_cust", I32, 0, self.rows["cust"] - 1)
+        self._prefixKey(t, "s_item", "item")
+        self._prefixKey(t, "s_cust", "cust")
         self._uniform(t, "s_nation", I32, 0, self.rows["nation"] - 1)
         self._uniform(t, "s_qty", I32, 1, 50)
         self._decimal(t, "s_price", 90000, 10500000, 100)

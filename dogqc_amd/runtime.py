@@ -29,7 +29,7 @@ RUNTIME_SYMBOLS = [
 ]
 NOALLOC_WS_BYTES = 64 << 20  # DQC_NOALLOC_WS_BYTES
 GEN_SYMBOLS = ["dqc_gen_uniform", "dqc_gen_decimal", "dqc_gen_pick_u32", "dqc_gen_affine_perm",
-               "dqc_gen_inverse_cdf", "dqc_gen_text"]
+               "dqc_gen_inverse_cdf", "dqc_gen_text", "dqc_gen_prefix_key"]
 PLAN_SYMBOLS = ["dqc_plan_describe", "dqc_plan_layout_of", "dqc_plan_enqueue"]
 
 

@@ -42,6 +42,17 @@ int dqc_gen_pick_u32(uint32_t* out, int64_t n, uint64_t seed, uint32_t column, c
  * (a, b < n < 2^31). */
 int dqc_gen_affine_perm(int32_t* out, int64_t n, uint64_t a, uint64_t b, void* stream);
 
+/* Foreign keys whose every prefix stays inside the referenced table's
+ * prefix: out[i] = (a * r + b) mod nkeys, the key of row r of a table whose
+ * keys are dqc_gen_affine_perm(nkeys, a, b), with r uniform over that
+ * table's first ceil(nkeys * (i + 1) / n) rows.  A query that reads the
+ * first f * n rows of this column and the first f * nkeys rows of the
+ * referenced table then finds every key, so join selectivities are the same
+ * at every prefix size (low rows are referenced more often: row r by about
+ * (n / nkeys) * ln(nkeys / r) rows).  nkeys < 2^31, a, b < nkeys. */
+int dqc_gen_prefix_key(int32_t* out, int64_t n, uint64_t seed, uint32_t column, int64_t nkeys, uint64_t a,
+                       uint64_t b, void* stream);
+
 /* out[i] = values[r], r the first index with cdf[r] > u (u uniform in
  * [0,1) from 53 random bits): inverse-CDF sampling of a discrete
  * distribution (Zipf) given its cumulative table. */

@@ -36,6 +36,16 @@ def uniform(r, lo, hi):
     return lo + (((r >> np.uint64(32)) * rng) >> np.uint64(32)).astype(np.int64)
 
 
+def prefixKeys(r, n, nkeys):
+    """dqc_gen_prefix_key: the key of row floor(u * ceil(nkeys * (i+1) / n))
+    of a table keyed by the affine permutation of permParams(nkeys)."""
+    i = np.arange(n, dtype=np.int64)
+    rows = np.clip((nkeys * (i + 1) + n - 1) // n, 1, nkeys).astype(np.uint64)
+    row = ((r >> np.uint64(32)) * rows) >> np.uint64(32)
+    a, c = datagen.permParams(nkeys)
+    return ((np.uint64(a) * row + np.uint64(c)) % np.uint64(nkeys)).astype(np.int32)
+
+
 _TABLES = {}
 
 
@@ -64,8 +74,16 @@ def test_datagen_matches_restatement():
     assert np.array_equal(date, workloads.dateTable()[uniform(rnd(seed, cid[("cfg0", "date")], rows), 0, 2556)])
     nf = T.rows["fact"]
     fk = T.store[("fact", "f_fk", None)].toArray(np.int32, nf)
-    assert np.array_equal(fk, uniform(rnd(seed, cid[("fact", "f_fk")], np.arange(nf)), 0, T.rows["dim"] - 1)
-                          .astype(np.int32))
+    assert np.array_equal(fk, prefixKeys(rnd(seed, cid[("fact", "f_fk")], np.arange(nf)), nf, T.rows["dim"]))
+    # every prefix of a referencing table stays inside the same fraction of
+    # the referenced one: a query at any scale finds all its keys
+    for ref, t, a in (("dim", "fact", "f_fk"), ("item", "sales", "s_item"), ("cust", "sales", "s_cust")):
+        keys = T.store[(ref, {"dim": "d_key", "item": "i_key", "cust": "c_key"}[ref], None)].toArray(
+            np.int32, T.rows[ref])
+        fks = T.store[(t, a, None)].toArray(np.int32, T.rows[t])
+        for f in (0.01, 0.1, 0.37, 1.0):
+            nr, nt = max(1, int(round(T.rows[ref] * f))), max(1, int(round(T.rows[t] * f)))
+            assert np.isin(fks[:nt], keys[:nr]).all(), (t, a, f)
     # permutations are permutations
     for t, a in (("dim", "d_key"), ("item", "i_key"), ("cust", "c_key"), ("nation", "n_key")):
         k = T.store[(t, a, None)].toArray(np.int32, T.rows[t])
