@@ -796,6 +796,10 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.emit("%s = jd.x; %s = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);"
                   % (moff, mend, self.fecap))
         pipe.code.close()
+        if not outer:
+            # an empty bucket ends an inner probe row here, so the probe-side
+            # columns read below are not fetched for it
+            pipe.emit("if (%s >= %s) active[u] = 0;" % (moff, mend))
         pipe.need(probeLive)
         pipe.emit("int mact%d = active[u] && %s < %s;" % (J, moff, mend))
         nullReg = None
@@ -936,6 +940,9 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.emit("%s = jd.x; %s = jd.x < %s ? std::min<uint64_t>(jd.y, %s - jd.x) : 0;"
                   % (moff, mlen, self.fecap, self.fecap))
         pipe.code.close()
+        # an empty bucket ends the (inner) probe row here, so the probe-side
+        # columns read below are not fetched for it
+        pipe.emit("if (%s == 0) active[u] = 0;" % mlen)
         pipe.need(probeLive)
         # everything the matching and the operators above read from the probe
         # tuple moves with it to the lanes that process its matches
