@@ -9,7 +9,8 @@ relationalAlgebra.py:453-455) plus the MI355X-specific ones:
     workgroup size, rows per thread
   run time (dqc_plan_tuning, no recompilation):
     grid size per pipeline, refill threshold, LDS aggregation slots and
-    replicas, hash-table size factor
+    replicas (pipelines annotated LDSn), hash-table size factor, and the
+    size of the LDS join filter together with its probe pipeline's grid
 
 Each candidate is timed with HIP events around every launch (median of a few
 runs), so one pipeline's knob is judged by that pipeline's kernel time; the
@@ -158,9 +159,40 @@ class Autotuner(object):
             trySet("grid", pid, [cus * k for k in (1, 2, 4, 8, 16, 32)], pipe)
             if any(a.startswith("LR") for a in pipe["annotations"]):
                 trySet("refill_threshold", pid, [8, 16, 32, 48, 64], pipe)
+            if any(a.startswith("LDS") for a in pipe["annotations"]):
+                # LDS aggregation width: table slots (-1: no LDS tier), then
+                # accumulator replicas
+                trySet("lds_slots", pid, [-1, 256, 1024, 4096], pipe)
+                if tuning.lds_slots[pid] >= 0:
+                    trySet("lds_replicas", pid, [1, 4, 16, 64], pipe)
         for ht in module.desc["hts"]:
-            if ht["kind"] in ("unique", "multi", "agg"):
+            if ht["kind"] in ("unique", "agg"):
                 trySet("ht_factor", ht["index"], [0.5, 0.75, 1.5, 2.0, 4.0], None)
+            elif ht["kind"] == "multi":
+                # a multimap has a bucket per build row by default; keys that
+                # repeat leave most buckets empty, and a smaller directory
+                # stays in L2
+                trySet("ht_factor", ht["index"], [0.125, 0.25, 0.5, 0.75, 1.5], None)
+        # the LDS join filter (pipelines annotated BFn) is copied into LDS by
+        # every workgroup of its probe pipeline, so it can only pay with few,
+        # large workgroups: its size and those pipelines' grids are searched
+        # together, judged by the whole plan (the build sets the filter)
+        bfPipes = [p["id"] for p in self._pipelines(module) if any(a.startswith("BF") for a in p["annotations"])]
+        if bfPipes:
+            keep = (tuning.join_filter, [tuning.grid[p] for p in bfPipes])
+            bestScore = score(best)
+            for jf in (32, 64, 96, 128):
+                for g in (cus, 2 * cus, 4 * cus):
+                    tuning.join_filter = jf
+                    for p in bfPipes:
+                        tuning.grid[p] = g
+                    times, fits = m.measure(tuning)
+                    self.log.append(("join_filter", jf, g, score(times)))
+                    if fits and score(times) < bestScore * 0.98:
+                        keep, bestScore, best = (jf, [g] * len(bfPipes)), score(times), times
+            tuning.join_filter = keep[0]
+            for p, g in zip(bfPipes, keep[1]):
+                tuning.grid[p] = g
         return best
 
     def run(self, store=None, rows=None):
@@ -282,7 +314,7 @@ def tuneServing(plans, scales=(0.1, 1.0, 10.0), tableScale=10.0, reps=5, out=Non
         config = planlib.PLANS[name][0]
         for sc in scales:
             rows = serving.prefixRows(tables, config, sc)
-            tuner = Autotuner(name, acc, reps=reps, verbose=False)
+            tuner = Autotuner(name, acc, reps=reps, verbose=True)
             best, allRecs = tuner.run(store=tables.store, rows=rows)
             best["scale"] = sc
             path = saveRecord(best, out)

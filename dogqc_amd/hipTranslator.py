@@ -1844,6 +1844,7 @@ class AggregationTranslator(UnaryTranslator):
         """Workgroup LDS table: W slots (runtime, power of two), accumulators
         replicated R times; flushed once per workgroup."""
         pid = pipe.pipeId
+        pipe.annotations.append("LDS%d" % self.algExpr.opId)
         accBytes = sum(CType.size[acc["ctype"]] for acc in self.accs)
         # LDS in use besides the table: static arrays, a join filter in front
         used = "(uint64_t)%du" % (staticLdsBytes(pipe) + 1024)
