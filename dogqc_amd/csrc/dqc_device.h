@@ -330,12 +330,66 @@ DQC_DEVICE_INLINE int64_t agg_slot_exact(uint64_t* tags, uint32_t cap, uint64_t 
     return agg_slot_exact_from(tags, cap, tag, slot_of(h, cap), 0u, status);
 }
 
+// Key columns of a hashed group table are written by the lane that claims
+// a slot and compared by lanes on every XCD while the same kernel runs.
+// Plain stores would stay in the writer's L2, which the other XCDs do not
+// see, and an agent-scope release / acquire pair writes back or invalidates
+// the whole L2 (buffer_wbl2 / buffer_inv) at every row.  So the keys are
+// stored and loaded as agent-scope atomics instead, which go through to the
+// coherence point (sc1), and the claiming lane waits for its key stores to
+// complete (vmcnt 0) before it publishes the tag.
+template <typename T>
+DQC_DEVICE_INLINE void coherent_store(T* p, T v) {
+    if constexpr (sizeof(T) % 8 == 0) {
+        uint64_t w[sizeof(T) / 8];
+        __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 8); ++i)
+            __hip_atomic_store((uint64_t*)p + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if constexpr (sizeof(T) == 4) {
+        uint32_t w;
+        __builtin_memcpy(&w, &v, 4);
+        __hip_atomic_store((uint32_t*)p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        static_assert(sizeof(T) == 1, "key columns are 1, 4 or 8n bytes");
+        uint8_t w;
+        __builtin_memcpy(&w, &v, 1);
+        __hip_atomic_store((uint8_t*)p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <typename T>
+DQC_DEVICE_INLINE T coherent_load(const T* p) {
+    T v;
+    if constexpr (sizeof(T) % 8 == 0) {
+        uint64_t w[sizeof(T) / 8];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 8); ++i)
+            w[i] = __hip_atomic_load((const uint64_t*)p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_memcpy(&v, w, sizeof(T));
+    } else if constexpr (sizeof(T) == 4) {
+        const uint32_t w = __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_memcpy(&v, &w, 4);
+    } else {
+        static_assert(sizeof(T) == 1, "key columns are 1, 4 or 8n bytes");
+        const uint8_t w = __hip_atomic_load((const uint8_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_memcpy(&v, &w, 1);
+    }
+    return v;
+}
+
+// Wait until this lane's vector memory operations have completed
+// (s_waitcnt vmcnt(0), gfx9 encoding: expcnt and lgkmcnt left unwaited).
+DQC_DEVICE_INLINE void wait_vector_memory() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Group slot in hashed mode.  The claiming lane CASes EMPTY->LOCKED, writes
-// the key columns through `write_keys(slot)`, then publishes the tag with a
-// release store; other lanes acquire-load the tag and compare keys through
-// `keys_equal(slot)`.  The claim branch precedes the re-check in program
-// order, so lanes of one wavefront waiting on a slot claimed by a sibling
-// lane always observe the publication on their next loop trip.
+// the key columns through `write_keys(slot)` (coherent_store), waits for
+// them, then publishes the tag; other lanes read the tag and compare keys
+// through `keys_equal(slot)` (coherent_load, issued only after the tag they
+// depend on has been read).  All are agent-scope atomics without fences.
+// The claim branch precedes the re-check in program order, so lanes of one
+// wavefront waiting on a slot claimed by a sibling lane always observe the
+// publication on their next loop trip.
 template <typename WriteKeys, typename KeysEqual>
 DQC_DEVICE_INLINE int64_t agg_slot_hashed(uint64_t* tags, uint32_t cap, uint64_t tag, uint64_t h,
                                           const uint32_t* status, WriteKeys write_keys, KeysEqual keys_equal) {
@@ -343,13 +397,14 @@ DQC_DEVICE_INLINE int64_t agg_slot_hashed(uint64_t* tags, uint32_t cap, uint64_t
     uint32_t n = 0;
     while (n < cap && n < MAX_PROBE) {
         if ((n & 31u) == 31u && overflow_raised(status)) return -1;
-        uint64_t t = __hip_atomic_load(&tags[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t t = __hip_atomic_load(&tags[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (t == TAG_EMPTY) {
             uint64_t expected = TAG_EMPTY;
-            if (__hip_atomic_compare_exchange_strong(&tags[s], &expected, TAG_LOCKED, __ATOMIC_ACQ_REL,
-                                                     __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (__hip_atomic_compare_exchange_strong(&tags[s], &expected, TAG_LOCKED, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 write_keys(s);
-                __hip_atomic_store(&tags[s], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                wait_vector_memory();
+                __hip_atomic_store(&tags[s], tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return (int64_t)s;
             }
             continue;  // lost the race: re-read this slot
@@ -628,18 +683,73 @@ DQC_DEVICE_INLINE str_offs to_string_offset(const char* chars, str_t s) {
     return o;
 }
 
+// Bytes [p, p + n), n <= 8, as a little-endian word with zeros above byte n.
+// Reads the one or two aligned 8-byte words holding those bytes: an aligned
+// word that holds a byte of a buffer lies inside that buffer's allocation
+// (device allocations and code-object sections are 8-byte aligned or more).
+DQC_DEVICE_INLINE uint64_t load_bytes(const char* p, uint32_t n) {
+    const uintptr_t a = (uintptr_t)p & ~(uintptr_t)7;
+    const uint32_t off = (uint32_t)((uintptr_t)p & 7u);
+    uint64_t w = *(const uint64_t*)a >> (8u * off);
+    if (off + n > 8u) w |= *(const uint64_t*)(a + 8u) << (64u - 8u * off);
+    return n >= 8u ? w : (w & ((1ull << (8u * n)) - 1ull));
+}
+
+// Byte equality, 8 bytes per step.
 DQC_DEVICE_INLINE bool string_equals(str_t a, str_t b) {
     const int64_t n = a.end - a.start;
     if (n != b.end - b.start) return false;
-    for (int64_t i = 0; i < n; ++i)
-        if (a.start[i] != b.start[i]) return false;
+    for (int64_t o = 0; o < n; o += 8) {
+        const uint32_t m = n - o < 8 ? (uint32_t)(n - o) : 8u;
+        if (load_bytes(a.start + o, m) != load_bytes(b.start + o, m)) return false;
+    }
     return true;
 }
 
+// String hash over 8-byte chunks: the chunks' mixes are summed (chunk k
+// mixed with its index), so any split of the chunks over lanes followed by
+// a sum gives the same value (string_hash_wave).
+DQC_DEVICE_INLINE uint64_t string_chunk_mix(uint64_t w, uint64_t k) {
+    return mix64(w ^ ((k + 1u) * 0x9E3779B97F4A7C15ull));
+}
+DQC_DEVICE_INLINE uint64_t string_hash_final(uint64_t sum, uint64_t n) {
+    return mix64(sum ^ mix64(n + 0x84222325cbf29ce4ull));
+}
 DQC_DEVICE_INLINE uint64_t string_hash(str_t s) {
-    uint64_t h = 0x84222325cbf29ce4ull;
-    for (const char* c = s.start; c < s.end; ++c) h = (h ^ (uint8_t)*c) * 0x100000001b3ull;  // FNV-1a
-    return mix64(h);
+    const uint64_t n = (uint64_t)(s.end - s.start);
+    uint64_t sum = 0;
+#pragma unroll 4
+    for (uint64_t o = 0; o < n; o += 8) {
+        const uint32_t m = n - o < 8u ? (uint32_t)(n - o) : 8u;
+        sum += string_chunk_mix(load_bytes(s.start + o, m), o >> 3);
+    }
+    return string_hash_final(sum, n);
+}
+
+// Wavefront-cooperative string hash (the push-down form of the reference's
+// stringHashPushDown, hashing.cuh:39-62): the strings of the active lanes
+// are hashed one after another by all 64 lanes, lane l taking the chunks
+// l, l + 64, ...; a wavefront sum combines them.  The value equals
+// string_hash's.  Called by every lane of the wavefront (uniform control
+// flow); inactive lanes get 0.
+DQC_DEVICE_INLINE uint64_t string_hash_wave(bool active, str_t s) {
+    const uint32_t lane = lane_id();
+    uint64_t result = 0;
+    uint64_t todo = ballot(active);
+    while (todo) {
+        const int t = first_lane(todo);
+        todo &= todo - 1u;
+        const str_t h = shfl(s, t);
+        const uint64_t n = (uint64_t)(h.end - h.start);
+        uint64_t part = 0;
+        for (uint64_t o = 8u * lane; o < n; o += 8u * WAVE) {
+            const uint32_t m = n - o < 8u ? (uint32_t)(n - o) : 8u;
+            part += string_chunk_mix(load_bytes(h.start + o, m), o >> 3);
+        }
+        const uint64_t sum = wave_sum(part);
+        if ((int)lane == t) result = string_hash_final(sum, n);
+    }
+    return result;
 }
 
 // SQL SUBSTRING(s FROM from FOR len), 1-based, clipped to the string
@@ -756,18 +866,26 @@ DQC_DEVICE_INLINE const char* string_find(const char* from, const char* end, con
     const char* last = end - L;
     uint64_t win = 0;
     uint32_t have = 0;
-    for (const char* p = (const char*)((uintptr_t)from & ~(uintptr_t)7); p < end; p += 8) {
-        const uint64_t w = *(const uint64_t*)p;
+    // the string's aligned words are loaded 8 at a time (independent loads,
+    // one round trip for up to 64 bytes), then scanned byte by byte
+    constexpr int BATCH = 8;
+    for (const char* p0 = (const char*)((uintptr_t)from & ~(uintptr_t)7); p0 < end; p0 += 8 * BATCH) {
+        uint64_t w[BATCH];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const char* q = p + j;
-            if (q >= from && q < end) {
-                win = (win << 8) | ((w >> (8 * j)) & 0xffu);
-                ++have;
-                if (have >= H && (win & hmask) == head) {
-                    const char* st = q - (H - 1);
-                    if (st > last) return nullptr;
-                    if (L <= H || bytes_equal(st + H, lit + H, L - H)) return st + L;
+        for (int i = 0; i < BATCH; ++i) w[i] = p0 + 8 * i < end ? *(const uint64_t*)(p0 + 8 * i) : 0ull;
+#pragma unroll
+        for (int i = 0; i < BATCH; ++i) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const char* q = p0 + 8 * i + j;
+                if (q >= from && q < end) {
+                    win = (win << 8) | ((w[i] >> (8 * j)) & 0xffu);
+                    ++have;
+                    if (have >= H && (win & hmask) == head) {
+                        const char* st = q - (H - 1);
+                        if (st > last) return nullptr;
+                        if (L <= H || bytes_equal(st + H, lit + H, L - H)) return st + L;
+                    }
                 }
             }
         }

@@ -155,6 +155,7 @@ class Pipeline(object):
         self.probedJoins = []
         self.nullFlags = {}         # attribute id -> null flag (outer-join build attributes)
         self.loopOps = 0            # operators so far that reassign tuples to lanes
+        self.uniqueCount = 0
         self.param("n", "int64_t", source["rows"])
         self.devRows = source.get("devRows")   # device-side row count (temp-table scans)
         self.tid = "tid_" + ident(source["tidName"])
@@ -206,6 +207,11 @@ class Pipeline(object):
 
     def has(self, attr):
         return attr.id in self.attrs
+
+    def uniqueId(self):
+        """A number not used before in this pipeline (names of locals)."""
+        self.uniqueCount += 1
+        return self.uniqueCount
 
     def access(self, attr):
         st = self.attrs.get(attr.id)
@@ -339,6 +345,7 @@ class Program(object):
         self.tables = OrderedDict()      # table name -> index
         self.columns = OrderedDict()     # (table, attr, part) -> dict(index, ctype)
         self.strings = OrderedDict()
+        self.structDecls = []             # record types of multimap entries
         self.likes = OrderedDict()        # LIKE pattern -> matcher function name
         self.likeDecls = []
         self.hts = []
@@ -465,6 +472,8 @@ class Program(object):
         matchers = [likeMatcherDecl(name, pattern, lit) for name, pattern, lit in self.likeDecls]
         for token, name in self.strings.items():
             out.emit(stringConstantDecl(name, token))
+        for decl in self.structDecls:
+            out.emit(decl)
         out.emit("")
         for m in matchers:
             out.extend(_asCode(m))
@@ -917,10 +926,17 @@ def firstUseFractions(scan):
     before that operator.  Attributes first read after a per-lane loop
     (multimatch probe, lane refill, nested loop) are left out: they are
     read per match, not per row."""
+    return useFractions(scan, set(scan.algExpr.outRelation), list(scan.algExpr.outRelation.values()))
+
+
+def useFractions(start, mine, terminal):
+    """First-use fractions of the attribute ids `mine`, walking the plan up
+    from translator `start` (the rows there count as 1); a sink operator
+    (aggregation, materialisation, projection) reads the attributes
+    `terminal`."""
     from dogqc_amd.hashJoins import EquiJoinTranslator
     from dogqc_amd.hipDivergenceBuffer import LaneRefill
     from dogqc_amd.relationalAlgebra import Join
-    mine = set(scan.algExpr.outRelation)
     out = {}
 
     def use(attrs, frac):
@@ -935,7 +951,7 @@ def firstUseFractions(scan):
         return found
 
     frac = 1.0
-    t = scan
+    t = start
     while t.parent is not None:
         p = t.parent
         a = p.algExpr
@@ -946,7 +962,7 @@ def firstUseFractions(scan):
             frac *= float(getattr(a, "selectivity", 1.0))
         elif isinstance(p, EquiJoinTranslator):
             if p.leftChild is t:          # build side: everything is stored here
-                use(list(scan.algExpr.outRelation.values()), frac)
+                use(terminal, frac)
                 break
             use(list(a.probeKeyAttributes.values()), frac)
             if a.multimatch and a.joinType in (Join.INNER, Join.OUTER):
@@ -957,7 +973,7 @@ def firstUseFractions(scan):
         elif isinstance(p, MapTranslator):
             use(exprAttrs(a.expression), frac)
         elif isinstance(p, (AggregationTranslator, MaterializeTranslator, ProjectionTranslator)):
-            use(list(scan.algExpr.outRelation.values()), frac)
+            use(terminal, frac)
             break
         t = p
     return out
@@ -1500,13 +1516,25 @@ class AggregationTranslator(UnaryTranslator):
                 for acc, v in zip(self.accs, vals):
                     pipe.emit(_guarded(acc, _combine(acc["kind"], acc["reg"], v)))
                 pipe.code.close()
-            # end of kernel: wave reduction + one atomic per wavefront
+            # end of kernel: wave reduction, the waves' partials combined in
+            # LDS, one atomic per workgroup (thousands of atomics on one
+            # address serialise at its memory channel)
+            nw = pipe.bs // 64
             for acc in self.accs:
                 red = {"sum": "wave_sum", "count": "wave_sum", "min": "wave_min", "max": "wave_max"}[acc["kind"]]
+                pipe.staticLds.append("__shared__ %s wred%d_%d[%d];" % (acc["ctype"], a.opId, acc["i"], nw))
                 pipe.epilogue.emit("%s = dqc::%s(%s);" % (acc["reg"], red, acc["reg"]))
             pipe.epilogue.open("if (lane == 0)")
             for acc in self.accs:
-                pipe.epilogue.emit("%s;" % _atomic(acc["kind"], acc["field"] + "[0]", acc["reg"]))
+                pipe.epilogue.emit("wred%d_%d[wave] = %s;" % (a.opId, acc["i"], acc["reg"]))
+            pipe.epilogue.close()
+            pipe.epilogue.emit("__syncthreads();")
+            pipe.epilogue.open("if (threadIdx.x == 0)")
+            for acc in self.accs:
+                w = "wred%d_%d" % (a.opId, acc["i"])
+                pipe.epilogue.emit("for (uint32_t w = 1; w < %du; ++w) %s" % (nw, _combine(acc["kind"], w + "[0]",
+                                                                                          w + "[w]")))
+                pipe.epilogue.emit("%s;" % _atomic(acc["kind"], acc["field"] + "[0]", w + "[0]"))
             pipe.epilogue.close()
             return
 
@@ -1575,13 +1603,18 @@ class AggregationTranslator(UnaryTranslator):
                 keyFields.append(pipe.param("aht%d_k%d" % (a.opId, k), "%s*" % ct, "B_" + b))
             self.keyFields = keyFields
             with pipe.rowLoop():
+                pre = hoistStringHashes(pipe, groups)
                 pipe.code.open("if (active[u])")
                 pipe.emit("uint64_t gh = 0;")
                 for g in groups:
-                    pipe.emit("gh = dqc::hash_combine(gh, %s);" % _keyBits(g, pipe.access(g)))
+                    pipe.emit("gh = dqc::hash_combine(gh, %s);" % _keyBits(g, pipe.access(g), pre))
                 pipe.emit("const uint64_t gtag = dqc::TAG_TOP | (gh >> 1);")
-                writes = " ".join("%s[s] = %s;" % (f, pipe.access(g)) for f, g in zip(keyFields, groups))
-                eqs = " && ".join(_keyEq(g, "%s[s]" % f, pipe.access(g)) for f, g in zip(keyFields, groups))
+                # keys written and compared during the kernel by lanes of
+                # every XCD: coherent (agent-scope atomic) stores and loads
+                writes = " ".join("dqc::coherent_store(&%s[s], %s);" % (f, pipe.access(g))
+                                  for f, g in zip(keyFields, groups))
+                eqs = " && ".join(_keyEq(g, "dqc::coherent_load(&%s[s])" % f, pipe.access(g))
+                                  for f, g in zip(keyFields, groups))
                 pipe.emit("const int64_t gs = dqc::agg_slot_hashed(%s, %s, gtag, gh, &st->flags, [&](uint32_t s) { %s }, "
                           "[&](uint32_t s) { return %s; });" % (ftags, fcap, writes, eqs))
                 pipe.code.open("if (gs >= 0)")
@@ -2046,9 +2079,35 @@ def _ldsOp(kind, ptr, v):
     return "dqc::%s(%s, %s);" % (fn, ptr, v)
 
 
-def _keyBits(attr, reg):
+class StringHash(object):
+    """How string keys are hashed: by each lane for its own row (default),
+    or, with ``wave``, by the whole wavefront one string at a time
+    (dqc::string_hash_wave; the reference's stringHashPushDown,
+    hashing.cuh:39-62, which hashTableUtil.py:20 leaves commented out).  Both
+    give the same hash value, so tables built one way are probed the other."""
+    wave = False
+
+
+def hoistStringHashes(pipe, attrs):
+    """With StringHash.wave, emit the wavefront string hash of every STRING
+    attribute of `attrs` for the current row, at the current (wavefront-
+    uniform) position; returns {attr id: hash register} for _keyBits."""
+    pre = {}
+    if not StringHash.wave:
+        return pre
+    for a in attrs:
+        if a.dataType == Type.STRING and a.id not in pre:
+            name = "wsh%d_%d" % (a.id, pipe.uniqueId())
+            pipe.emit("const uint64_t %s = dqc::string_hash_wave(active[u], %s);" % (name, pipe.access(a)))
+            pre[a.id] = name
+    return pre
+
+
+def _keyBits(attr, reg, pre=None):
     t = attr.dataType
     if t == Type.STRING:
+        if pre and attr.id in pre:
+            return pre[attr.id]
         return "dqc::string_hash(%s)" % reg
     if t == Type.DOUBLE or t == Type.FLOAT:
         return "(uint64_t)__double_as_longlong((%s) == 0 ? 0.0 : (double)(%s))" % (reg, reg)
