@@ -254,7 +254,8 @@ def benchQ10(args):
     the query of the reference's only published measurements (sample
     profiler reports: krnl_lineitem7 15.2 ms divergent, 5.1 ms with lane
     refill at opId 9; RTX 2080, FP32 decimals, grid 100 x 32).  The refill
-    runs with the LDS queue and with the register buffer.  All plans
+    runs with the LDS queue and with the register buffer, and once more with
+    the group-by's string keys hashed by the whole wavefront.  All plans
     carry the lane-activity profile after the lineitem join (opId 9);
     kernel times are taken with the profile off, the histogram in one more
     run with it on.  Every plan's result is checked against the oracle."""
@@ -278,7 +279,7 @@ def benchQ10(args):
            "plans": {}}
     results = {}
     stream = rt.Stream()
-    for name in ("tpch10_p9", "tpch10_lr9_p9", "tpch10_lr9reg_p9"):
+    for name in ("tpch10_p9", "tpch10_lr9_p9", "tpch10_lr9reg_p9", "tpch10_ws_lr9_p9"):
         mod = rt.PlanModule(planlib.translate(name).compile(name))
         inputs = rt.BoundInputs(mod, acc, store=store)
         tun = rt.newTuning()
@@ -313,6 +314,7 @@ def benchQ10(args):
         print("[q10] %s: lineitem7 %.3f ms, total %.3f ms, mean active lanes %.1f of 64 over %d wave trips"
               % (name, per["krnl_lineitem7"], sum(per.values()), mean, iters), file=sys.stderr, flush=True)
     a, b, c = results["tpch10_p9"], results["tpch10_lr9_p9"], results["tpch10_lr9reg_p9"]
+    d = results["tpch10_ws_lr9_p9"]
     from oracle.planeval import evaluate
     alg, p, cfg, _ = planlib.build("tpch10", acc)
     plan = alg.resolveAlgebraPlan(p, cfg)
@@ -320,7 +322,8 @@ def benchQ10(args):
     out["parity"] = {"oracle": "planeval", "groups": int(len(want["c_custkey"])),
                      "plain_ok": sameGroups(a, want, ["c_custkey"], ["revenue"]),
                      "refill_ok": sameGroups(b, want, ["c_custkey"], ["revenue"]),
-                     "register_refill_ok": sameGroups(c, want, ["c_custkey"], ["revenue"])}
+                     "register_refill_ok": sameGroups(c, want, ["c_custkey"], ["revenue"]),
+                     "wave_string_hash_ok": sameGroups(d, want, ["c_custkey"], ["revenue"])}
     best = min(out["plans"].values(), key=lambda r: r["total_ms"])
     out["value"] = li / (best["total_ms"] * 1e-3)
     out["ms_per_step"] = best["total_ms"]

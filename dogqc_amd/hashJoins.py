@@ -42,6 +42,19 @@ from dogqc_amd.types import Type
 from dogqc_amd.hiplang import CType, PACK_BITS
 
 
+class StoredField(object):
+    """One stored build column: column ``name[ix]`` of a unique table's
+    structure of arrays, or field k of a multimap's entry records,
+    ``name[ix].fk`` (a probe that finds a short match list then reads its
+    keys and payloads from one cache line rather than one per column)."""
+
+    def __init__(self, name, k=None):
+        self.name, self.k = name, k
+
+    def at(self, ix):
+        return "%s[%s]" % (self.name, ix) if self.k is None else "%s[%s].f%d" % (self.name, ix, self.k)
+
+
 class EquiJoinTranslator(BinaryTranslator):
 
     usePushDownJoin = False
@@ -104,9 +117,19 @@ class EquiJoinTranslator(BinaryTranslator):
         self.filtered = (not a.multimatch and a.joinType == Join.INNER
                          and float(a.selectivity) <= self.FILTER_SELECTIVITY)
 
-    def _hashCode(self, pipe, keys, tagVar, hashVar, tagIsReg=False):
+    def _preHash(self, pipe):
+        """Wavefront string hashes of the probe keys (StringHash.wave), at
+        the current uniform position; None when the keys are not hashed."""
+        if self.tag32 or self.exact:
+            return None
+        from dogqc_amd.hipTranslator import hoistStringHashes
+        return hoistStringHashes(pipe, self.probeKeys)
+
+    def _hashCode(self, pipe, keys, tagVar, hashVar, tagIsReg=False, pre=None):
         """Emit the tag and hash of a key tuple.  With tagIsReg the tag goes
-        into the per-row register ``tagVar`` (for two-phase lookups)."""
+        into the per-row register ``tagVar`` (for two-phase lookups); `pre`
+        holds string key hashes computed by the wavefront
+        (hoistStringHashes)."""
         tagDecl = tagVar if tagIsReg else "const %s %s" % (self.tagType, tagVar)
         if self.tag32:
             # packed key + 1; the key with all 32 bits set gets tag 0 (extra slot)
@@ -134,7 +157,7 @@ class EquiJoinTranslator(BinaryTranslator):
             from dogqc_amd.hipTranslator import _keyBits
             pipe.emit("uint64_t %s = 0;" % hashVar)
             for k in keys:
-                pipe.emit("%s = dqc::hash_combine(%s, %s);" % (hashVar, hashVar, _keyBits(k, pipe.access(k))))
+                pipe.emit("%s = dqc::hash_combine(%s, %s);" % (hashVar, hashVar, _keyBits(k, pipe.access(k), pre)))
             pipe.emit("%s = dqc::TAG_TOP | (%s >> 1);" % (tagDecl, hashVar))
 
     # ----------------------------------------------------------- build
@@ -180,11 +203,24 @@ class EquiJoinTranslator(BinaryTranslator):
             fecap = pipe.param("jht%d_ecap" % J, "uint32_t", "L.mment[%d]" % idx)
             count = "L.mment[%d]" % idx
         fields = []
-        for k, att in enumerate(self.stored):
-            ct = pipe.attrs[att.id].ctype
-            b = prog.buffer("jht%d_p%d" % (J, k), ct, count)
-            self.storedBuffers.append((b, ct))
-            fields.append(pipe.param("jht%d_p%d" % (J, k), "%s*" % ct, "B_" + b))
+        if kind == "unique":
+            for k, att in enumerate(self.stored):
+                ct = pipe.attrs[att.id].ctype
+                b = prog.buffer("jht%d_p%d" % (J, k), ct, count)
+                self.storedBuffers.append((b, ct))
+                fields.append(StoredField(pipe.param("jht%d_p%d" % (J, k), "%s*" % ct, "B_" + b)))
+        else:
+            # multimap entries are records of the stored columns, the widest
+            # first so that no field needs padding before it
+            self.storedTypes = [pipe.attrs[att.id].ctype for att in self.stored]
+            rec = "dqc_mm%d" % J
+            order = sorted(range(len(self.stored)), key=lambda k: (-CType.size[self.storedTypes[k]], k))
+            prog.structDecls.append("struct %s { %s };" % (
+                rec, " ".join("%s f%d;" % (self.storedTypes[k], k) for k in order) or "uint32_t unused;"))
+            self.entryRecord = rec
+            b = prog.buffer("jht%d_e" % J, rec, count)
+            fe = pipe.param("jht%d_e" % J, "%s*" % rec, "B_" + b)
+            fields = [StoredField(fe, k) for k in range(len(self.stored))]
         fbf = None
         if kind == "unique" and self.filtered:
             prog.hostVars["bfw%d" % idx] = (
@@ -252,7 +288,7 @@ class EquiJoinTranslator(BinaryTranslator):
                           "dqc::next_slot(%s, %s), 1u, &st->flags);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
             pipe.code.open("if (js >= 0)")
             for f, v in zip(ins["fields"], values):
-                pipe.emit("%s[js] = %s;" % (f, v))
+                pipe.emit("%s = %s;" % (f.at("js"), v))
             pipe.code.close()
             pipe.emit("else if (js == -1) { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
                       "atomicOr(&st->ht_overflow_mask, %du); }" % (1 << idx))
@@ -443,7 +479,7 @@ class EquiJoinTranslator(BinaryTranslator):
             pipe.code.open("if (js >= 0)")
             pipe.emit("%s[js] = %s;" % (ins["ftags"], jtag))
             for f, att, r in zip(fields, self.stored, pw):
-                pipe.emit("%s[js] = dqc::from_word<%s>(%s);" % (f, pipe.attrs[att.id].ctype, r))
+                pipe.emit("%s = dqc::from_word<%s>(%s);" % (f.at("js"), pipe.attrs[att.id].ctype, r))
             pipe.code.close()
             pipe.code.open("else if (spill)")
             pipe.emit("const uint32_t sp = atomicAdd(%s, 1u);" % fspc)
@@ -489,7 +525,7 @@ class EquiJoinTranslator(BinaryTranslator):
     def _storeEntry(self, pipe, idx, fecap, fields, pos):
         pipe.code.open("if (%s < %s)" % (pos, fecap))
         for f, att in zip(fields, self.stored):
-            pipe.emit("%s[%s] = %s;" % (f, pos, pipe.access(att)))
+            pipe.emit("%s = %s;" % (f.at(pos), pipe.access(att)))
         pipe.code.close()
         pipe.emit("else { dqc::raise_status(&st->flags, DQC_ST_MM_OVERFLOW); "
                   "atomicOr(&st->mm_overflow_mask, %du); }" % (1 << idx))
@@ -566,17 +602,24 @@ class EquiJoinTranslator(BinaryTranslator):
     # ----------------------------------------------------------- probe
     def _storedFields(self, pipe):
         J = self.algExpr.opId
+        if self.ht["kind"] != "unique":
+            fe = pipe.param("jht%d_e" % J, "const %s*" % self.entryRecord, "B_jht%d_e" % J)
+            return [StoredField(fe, k) for k in range(len(self.stored))]
         out = []
         for k, (b, ct) in enumerate(self.storedBuffers):
-            out.append(pipe.param("jht%d_p%d" % (J, k), "const %s*" % ct, "B_" + b))
+            out.append(StoredField(pipe.param("jht%d_p%d" % (J, k), "const %s*" % ct, "B_" + b)))
         return out
+
+    def _storedType(self, k):
+        return self.storedBuffers[k][1] if self.ht["kind"] == "unique" else self.storedTypes[k]
 
     def _bindBuild(self, pipe, fields, indexReg, nullReg=None):
         """Build-side attributes become lazy loads from the stored columns at
         the row's position ``indexReg[u]``; with ``nullReg`` (outer join) a
         null row reads 0 / the empty string instead."""
-        for f, att, (b, ct) in zip(fields, self.stored, self.storedBuffers):
-            load = "%s[%s[u]]" % (f, indexReg)
+        for k, (f, att) in enumerate(zip(fields, self.stored)):
+            ct = self._storedType(k)
+            load = f.at(indexReg + "[u]")
             if nullReg is not None:
                 zero = "dqc::str_t{nullptr, nullptr}" if ct == CType.STR_TYPE else "(%s)0" % ct
                 load = "(%s[u] ? %s : %s)" % (nullReg, zero, load)
@@ -596,9 +639,9 @@ class EquiJoinTranslator(BinaryTranslator):
             for b, p in zip(self.buildKeys, self.probeKeys):
                 f = byId[b.id]
                 if b.dataType == Type.STRING:
-                    parts.append("dqc::string_equals(%s[%s], %s)" % (f, ix, pipe.access(p)))
+                    parts.append("dqc::string_equals(%s, %s)" % (f.at(ix), pipe.access(p)))
                 else:
-                    parts.append("(%s[%s] == %s)" % (f, ix, pipe.access(p)))
+                    parts.append("(%s == %s)" % (f.at(ix), pipe.access(p)))
         if a.conditions is not None:
             buildIds = set(att.id for att in self.stored)
             condAttrs = []
@@ -643,8 +686,9 @@ class EquiJoinTranslator(BinaryTranslator):
                 # (independent loads), then the comparisons
                 with pipe.rowLoop():
                     pipe.emit("%s = -1;" % slot)
+                    pre = self._preHash(pipe)
                     pipe.code.open("if (active[u])")
-                    self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True)
+                    self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True, pre=pre)
                     pipe.emit("%s = dqc::probe_start(jh%d, %s);" % (jc, J, fcap))
                     if lbf is not None:
                         # rows the filter rules out load no probe window
@@ -656,9 +700,10 @@ class EquiJoinTranslator(BinaryTranslator):
             with pipe.rowLoop():
                 if needCheck:
                     pipe.emit("%s = -1;" % slot)
+                pre = self._preHash(pipe) if needCheck else None
                 pipe.code.open("if (active[u])")
                 if needCheck:
-                    self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True)
+                    self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True, pre=pre)
                     if lbf is not None:
                         pipe.emit("%s = p.jbfp%d == 0u || %s;" % (jf, J, passes))
                         pipe.code.open("if (%s)" % jf)
@@ -722,8 +767,9 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.indexRegs[posName] = "uint32_t"
         if a.joinType in (Join.SEMI, Join.ANTI):
             with pipe.rowLoop():
+                pre = self._preHash(pipe)
                 pipe.code.open("if (active[u])")
-                self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+                self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J, pre=pre)
                 pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
                 pipe.emit("int fm = 0;")
                 pipe.emit("const uint32_t jend = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);"
@@ -790,8 +836,9 @@ class EquiJoinTranslator(BinaryTranslator):
         mend = pipe.reg("mend%d" % J, "uint32_t", "0")
         pos = posName + "[u]"
         pipe.emit("%s = 0; %s = 0;" % (moff, mend))
+        pre = self._preHash(pipe)
         pipe.code.open("if (active[u])")
-        self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+        self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J, pre=pre)
         pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
         pipe.emit("%s = jd.x; %s = (uint32_t)std::min<uint64_t>((uint64_t)jd.x + jd.y, %s);"
                   % (moff, mend, self.fecap))
@@ -934,8 +981,9 @@ class EquiJoinTranslator(BinaryTranslator):
             pipe.code.lines.insert(0, "    " * (pipe.code.depth - 1) + "if constexpr (PHASE == 0) {")
             pipe.code.depth += 1
         pipe.emit("%s = 0; %s = 0;" % (moff, mlen))
+        pre = self._preHash(pipe)
         pipe.code.open("if (active[u])")
-        self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J)
+        self._hashCode(pipe, self.probeKeys, "jtag%d" % J, "jh%d" % J, pre=pre)
         pipe.emit("const uint2 jd = ((const uint2*)%s)[dqc::slot_of(jh%d, %s)];" % (fdir, J, fcap))
         pipe.emit("%s = jd.x; %s = jd.x < %s ? std::min<uint64_t>(jd.y, %s - jd.x) : 0;"
                   % (moff, mlen, self.fecap, self.fecap))

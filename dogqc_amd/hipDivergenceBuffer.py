@@ -50,8 +50,27 @@ class LaneRefill(UnaryTranslator):
         self.threshold = LaneRefill.threshold
         self.bufferType = LaneRefill.bufferType
 
+    # a column still to be loaded that at least this fraction of the
+    # released rows reads later in the pipeline loads right after the
+    # release, with the others: the lanes are dense there, and one round
+    # trip replaces one per operator that reads a column
+    EAGER_AFTER_REFILL = 0.5
+
     def produce(self, ctxt):
         self.child.produce(ctxt)
+
+    def _eagerAfterRelease(self, pipe):
+        from dogqc_amd.hipTranslator import useFractions
+        cand = {}
+        for id, st in pipe.attrs.items():
+            if not st.loaded and st.loader is not None and tuple(st.deps) == (pipe.tid,):
+                cand[id] = st.attr
+        if not cand:
+            return
+        fracs = useFractions(self, set(cand), list(cand.values()))
+        eager = [cand[id] for id, f in fracs.items() if f >= LaneRefill.EAGER_AFTER_REFILL]
+        if eager:
+            pipe.need(eager)
 
     def opId(self):
         return self.algExpr.opId + 100000
@@ -121,6 +140,7 @@ class LaneRefill(UnaryTranslator):
         # the popped entries are read before later pushes reuse their slots
         pipe.emit("dqc::wave_lds_sync();")
         point["cond"] = "%s >= (rflush%d ? 1u : %s)" % (cnt, X, thr)
+        self._eagerAfterRelease(pipe)
         self.parent.consume(ctxt)
         pipe.code.close()
         pipe.emit("active[u] = 0;")
@@ -167,6 +187,7 @@ class LaneRefill(UnaryTranslator):
         pipe.emit("%s -= rtake;" % cnt)
         pipe.code.close()
         point["cond"] = go
+        self._eagerAfterRelease(pipe)
         self.parent.consume(ctxt)
         pipe.emit("active[u] = 0;")
         pipe.emit("%s = %s >= %s || (rflush%d && %s > 0u);" % (go, cnt, thr, X, cnt))

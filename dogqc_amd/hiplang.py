@@ -26,7 +26,7 @@ class CType(object):
                 CHAR: "INT8_MAX"}
     lowestValue = {INT: "INT32_MIN", UINT: "0u", INT64: "INT64_MIN", FP32: "(-FLT_MAX)", FP64: "(-DBL_MAX)",
                    CHAR: "INT8_MIN"}
-    size = {INT: 4, UINT: 4, UINT64: 8, INT64: 8, FP32: 4, FP64: 8, CHAR: 1, STR_OFFS: 16, BOOL: 4}
+    size = {INT: 4, UINT: 4, UINT64: 8, INT64: 8, FP32: 4, FP64: 8, CHAR: 1, STR_TYPE: 16, STR_OFFS: 16, BOOL: 4}
 
 
 def langType(relType, decimalRepr):

@@ -17,19 +17,22 @@ from dogqc_amd.relationalAlgebra import RelationalAlgebra, Reduction
 from dogqc_amd.types import Type
 from dogqc_amd.dbio import MemoryAccessor
 from dogqc_amd.hiplang import CType
-from dogqc_amd.hipTranslator import HipCompiler
+from dogqc_amd.hipTranslator import HipCompiler, StringHash
 from dogqc_amd.hashJoins import EquiJoinTranslator
 from dogqc_amd.hipDivergenceBuffer import LaneRefill, BufferType
 from dogqc_amd.kernel import KernelCall
 
 
 @contextlib.contextmanager
-def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBuild=False, bufferType=None):
+def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBuild=False, bufferType=None,
+          waveStrings=False):
     old = (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
-           KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType)
+           KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType,
+           StringHash.wave)
     EquiJoinTranslator.usePushDownJoin = pushdown
     EquiJoinTranslator.usePartitionedBuild = partBuild
     LaneRefill.bufferType = bufferType or BufferType.SMEM
+    StringHash.wave = bool(waveStrings)
     if block is not None:
         KernelCall.defaultBlockSize = block
     if threshold is not None:
@@ -40,7 +43,8 @@ def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBu
         yield
     finally:
         (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
-         KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType) = old
+         KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType,
+         StringHash.wave) = old
 
 
 # ------------------------------------------------------------- feature plans
@@ -418,6 +422,12 @@ PLANS["tpch3_profiled"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7]
 # profile after the lineitem join (opId 9), without and with lane refill there
 PLANS["tpch10_p9"] = ("tpch", {"query": 10, "profilers": [9]})
 PLANS["tpch10_lr9_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9]})
+# TPC-H queries with string join or group keys, their strings hashed by the
+# whole wavefront (StringHash.wave) instead of lane by lane
+WAVE_STRING_TPCH = [4, 5, 7, 9, 10, 12, 16, 18, 21, 22]
+for _q in WAVE_STRING_TPCH:
+    PLANS["tpch%d_ws" % _q] = ("tpch", {"query": _q, "wave_strings": True})
+PLANS["tpch10_ws_lr9_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9], "wave_strings": True})
 # lane refill with the register buffer (BufferType.REG) in place of the LDS queue
 PLANS["tpch10_lr9reg_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9], "buffer_type": "reg"})
 PLANS["tpch3_profiled_reg"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7], "buffer_type": "reg"})
@@ -480,7 +490,8 @@ def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=
     with knobs(pushdown=opts.get("pushdown", False), block=block or opts.get("block"),
                threshold=opts.get("threshold"), rowsPerThread=rowsPerThread,
                partBuild=opts.get("part_build", False),
-               bufferType=BufferType.REG if opts.get("buffer_type") == "reg" else BufferType.SMEM):
+               bufferType=BufferType.REG if opts.get("buffer_type") == "reg" else BufferType.SMEM,
+               waveStrings=opts.get("wave_strings", False)):
         comp = HipCompiler(alg, "gfx950", decimalRepr)
         comp.setBuffers(opts.get("buffers", []))
         comp.setProfilers(opts.get("profilers", []))

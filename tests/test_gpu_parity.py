@@ -327,12 +327,14 @@ def test_tpch(q, sf):
     check("tpch%d" % q, tpchAcc(sf), storeKey=("tpch", sf), label="sf=%g" % sf)
 
 
-@pytest.mark.parametrize("name", ["tpch10_p9", "tpch10_lr9_p9", "tpch10_lr9reg_p9", "tpch3_profiled",
-                                  "tpch3_profiled_reg"])
+@pytest.mark.parametrize("name", ["tpch10_p9", "tpch10_lr9_p9", "tpch10_lr9reg_p9", "tpch10_ws_lr9_p9",
+                                  "tpch3_profiled", "tpch3_profiled_reg"]
+                         + ["tpch%d_ws" % q for q in planlib.WAVE_STRING_TPCH])
 @pytest.mark.parametrize("sf", [0.05, 0.002])
 def test_tpch_refill_variants(name, sf):
     """Q10 and Q3 with lane refill (LDS queue and register buffer) and
-    lane-activity profiles in the plan, against the oracle."""
+    lane-activity profiles in the plan, and the queries with string keys
+    hashed by the whole wavefront, against the oracle."""
     check(name, tpchAcc(sf), storeKey=("tpch", sf), label="sf=%g" % sf)
 
 
