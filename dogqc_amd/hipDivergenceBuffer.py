@@ -38,6 +38,7 @@ from dogqc_amd.translatorBase import UnaryTranslator
 class BufferType(Enum):
     SMEM = 1
     REG = 2
+    GLOBAL = 3
 
 
 class LaneRefill(UnaryTranslator):
@@ -94,10 +95,137 @@ class LaneRefill(UnaryTranslator):
                          "std::max<int32_t>(1, std::min<int32_t>(64, t->refill_threshold[%d] > 0 ? "
                          "t->refill_threshold[%d] : %d))" % (pid, pid, default))
         outer = pipe.refillPoints[-1] if pipe.refillPoints else None
+        if self.bufferType is BufferType.GLOBAL and self._globalPossible(pipe):
+            self._consumeGlobal(ctxt, pipe, X, live)
+            return
         if self.bufferType is BufferType.REG:
             self._consumeReg(ctxt, pipe, X, live, cnt, thr, outer)
         else:
             self._consumeSmem(ctxt, pipe, X, live, cnt, thr, outer)
+
+    def _globalPossible(self, pipe):
+        """The kernel split needs the first refill point of a pipeline that
+        is not launched in phases otherwise and ends in an aggregation or a
+        materialisation (a multimap build sink launches in phases itself)."""
+        from dogqc_amd.hipTranslator import AggregationTranslator, MaterializeTranslator, ProjectionTranslator
+        from dogqc_amd.hashJoins import EquiJoinTranslator
+        if pipe.phased or pipe.refillPoints or pipe.partAgg is not None:
+            return False
+        t = self
+        while t.parent is not None:
+            p = t.parent
+            if isinstance(p, EquiJoinTranslator) and p.leftChild is t:
+                return False
+            if isinstance(p, (AggregationTranslator, MaterializeTranslator, ProjectionTranslator)):
+                return True
+            t = p
+        return True
+
+    def _consumeGlobal(self, ctxt, pipe, X, live):
+        """Lane refill across a kernel boundary (BufferType.GLOBAL): the
+        pipeline launches twice.  PHASE 0 runs the operators below this
+        point and appends the survivors' live registers to a device queue
+        (one atomic per wavefront); PHASE 1 runs over the queue, every lane
+        one survivor, so the operators above start with full wavefronts and
+        a launch of their own (no LDS queue, no filter in LDS).  The queue
+        is sized from the plan's estimate; an overflow is reported through
+        split_count and the run repeated with the exact count."""
+        prog = ctxt.program
+        ht = prog.newHt("queue", X, "gq%d" % X)
+        idx = ht["index"]
+        est = prog.estimate(self.algExpr)
+        prog.hostVars["htcap%d" % idx] = "L.htcap[%d] = 0; L.mment[%d] = 0;" % (idx, idx)
+        prog.splitVars["scap%d" % idx] = (
+            "L.scap[%d] = t->split_capacity[%d] > 0 ? t->split_capacity[%d] : std::max<int64_t>(4096, %s + %s / 4);"
+            % (idx, idx, idx, est, est))
+        cap = "L.scap[%d]" % idx
+        fcap = pipe.param("gqcap%d" % X, "uint64_t", cap)
+        regs = []
+        for k, (r, ct) in enumerate(live.items()):
+            b = prog.buffer("gq%d_r%d" % (X, k), ct, cap)
+            regs.append((r, pipe.param("gq%d_r%d" % (X, k), "%s*" % ct, "B_" + b)))
+        pipe.annotations.append("GQ%d" % X)
+        # survivors are staged per wavefront in LDS and appended with one
+        # atomic per stage flush (one per wavefront and trip on a single
+        # counter serialises there: c3 took 10.9 ms)
+        from dogqc_amd.hiplang import CType
+        NW = pipe.bs // 64
+        rowBytes = sum(CType.size.get(ct, 8) for _, ct in live.items())
+        SW = max(64, min(128, (2048 // max(1, rowBytes)) // 64 * 64))
+        for k, (r, ct) in enumerate(live.items()):
+            pipe.staticLds.append("__shared__ %s gqs%d_%d[%d][%d];" % (ct, X, k, NW, SW))
+        pipe.staticLds.append("__shared__ uint32_t gqs%d_blk[%d];" % (X, NW))
+        pipe.staticLds.append("__shared__ unsigned long long gqs%d_base;" % X)
+        gqn = pipe.reg("gqn%d" % X, "uint32_t", "0u", perRow=False)
+        counter = "st->split_count[%d]" % idx
+        # everything emitted so far is PHASE 0
+        pipe.code.lines.insert(0, "    " * (pipe.code.depth - 1) + "if constexpr (PHASE == 0) {")
+        pipe.code.depth += 1
+        pipe.emit("// lane refill across launches (opId %d): append the survivors" % X)
+        pipe.code.open("")
+        pipe.emit("const uint64_t qm = dqc::ballot(active[u]);")
+        pipe.emit("const uint32_t qn = dqc::popc64(qm);")
+        pipe.code.open("if (%s + qn > %du)" % (gqn, SW))
+        pipe.emit("dqc::wave_lds_sync();")
+        pipe.emit("unsigned long long qb = 0;")
+        pipe.emit("if (lane == 0u) qb = atomicAdd((unsigned long long*)&%s, (unsigned long long)%s);"
+                  % (counter, gqn))
+        pipe.emit("qb = dqc::uniform64(dqc::shfl(qb, 0));")
+        pipe.code.open("for (uint32_t i = lane; i < %s; i += 64u)" % gqn)
+        pipe.code.open("if (qb + i < %s)" % fcap)
+        for k, (r, f) in enumerate(regs):
+            pipe.emit("%s[qb + i] = gqs%d_%d[wave][i];" % (f, X, k))
+        pipe.code.close()
+        pipe.code.close()
+        pipe.emit("dqc::wave_lds_sync();")
+        pipe.emit("%s = 0u;" % gqn)
+        pipe.code.close()
+        pipe.code.open("if (active[u])")
+        pipe.emit("const uint32_t sp = %s + dqc::lanes_below(qm);" % gqn)
+        for k, (r, f) in enumerate(regs):
+            pipe.emit("gqs%d_%d[wave][sp] = %s[u];" % (X, k, r))
+        pipe.code.close()
+        pipe.emit("%s += qn;" % gqn)
+        pipe.emit("active[u] = 0;")
+        pipe.code.close()
+        pipe.code.close()   # end of PHASE 0
+        # end of the first launch: the workgroup's staged survivors, one
+        # atomic per workgroup
+        e = pipe.epilogue
+        e.open("if constexpr (PHASE == 0)")
+        e.emit("dqc::wave_lds_sync();")
+        e.emit("if (lane == 0u) gqs%d_blk[wave] = %s;" % (X, gqn))
+        e.emit("__syncthreads();")
+        e.open("if (threadIdx.x == 0u)")
+        e.emit("uint32_t tot = 0u;")
+        e.emit("for (uint32_t w = 0; w < %du; ++w) tot += gqs%d_blk[w];" % (NW, X))
+        e.emit("gqs%d_base = tot ? atomicAdd((unsigned long long*)&%s, (unsigned long long)tot) : 0ull;"
+               % (X, counter))
+        e.close()
+        e.emit("__syncthreads();")
+        e.emit("uint64_t qwb = gqs%d_base;" % X)
+        e.emit("for (uint32_t w = 0; w < wave; ++w) qwb += gqs%d_blk[w];" % X)
+        e.open("for (uint32_t i = lane; i < %s; i += 64u)" % gqn)
+        e.open("if (qwb + i < %s)" % fcap)
+        for k, (r, f) in enumerate(regs):
+            e.emit("%s[qwb + i] = gqs%d_%d[wave][i];" % (f, X, k))
+        e.close()
+        e.close()
+        e.close()
+        pipe.code.open("else")
+        pipe.emit("// the loop index is a queue entry")
+        pipe.emit("const uint64_t qi = (uint64_t)%s[u];" % pipe.tid)
+        pipe.code.open("if (active[u])")
+        for r, f in regs:
+            pipe.emit("%s[u] = %s[qi];" % (r, f))
+        pipe.code.close()
+        pipe.code.close()
+        pipe.phased = True
+        pipe.phaseBetween = []
+        pipe.globalRefill = True
+        pipe.phaseRows[1] = (cap, "std::min<uint64_t>(p.st->split_count[%d], p.gqcap%d)" % (idx, X))
+        self._eagerAfterRelease(pipe)
+        self.parent.consume(ctxt)
 
     def _flushFlag(self, pipe, X, outer):
         if outer is None:

@@ -30,8 +30,11 @@ class LaneActivityProfiler(UnaryTranslator):
         f = pipe.param("prof", "unsigned long long*", "(unsigned long long*)B_profile")
         on = pipe.param("prof_on", "int32_t", "t->profile")
         pipe.emit("// lane activity profile (opId %d)" % P)
+        # after a kernel-split lane refill the first launch has no rows
+        # here: only the second launch's trips are counted
+        cond = on if not getattr(pipe, "globalRefill", False) else "PHASE == 1 && %s" % on
         with pipe.rowLoop():
-            pipe.code.open("if (%s)" % on)
+            pipe.code.open("if (%s)" % cond)
             pipe.emit("const uint64_t pm = dqc::ballot(active[u]);")
             pipe.emit("if (lane == 0) atomicAdd(&%s[%d * DQC_PROFILE_BUCKETS + dqc::popc64(pm)], 1ull);" % (f, slot))
             pipe.code.close()

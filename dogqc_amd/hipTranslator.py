@@ -740,8 +740,14 @@ def pipelineLaunch(prog, pipe, phase=None, rowsExpr=None, perCu=8):
             lines.append("if (part) kp.lw = 0u;")
         keep = "(size_t)0" if pipe.ldsFilter is None else "(size_t)(%s)" % pipe.ldsFilter["host"]
         lds = "(part ? %s : %s)" % (keep, lds)
-    lines.append("const unsigned g = gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, %s);"
-                 % (pipe.pipeId, pipe.pipeId, pipe.defaultGrid, rows, pipe.bs * pipe.U, per))
+    if phase == 1 and getattr(pipe, "globalRefill", False):
+        # the survivors' launch sizes its grid from the queue; the tuned grid
+        # belongs to the first launch (with an LDS join filter, a few large
+        # workgroups there)
+        lines.append("const unsigned g = gridFor(0, %s, %d, %s);" % (rows, pipe.bs * pipe.U, per))
+    else:
+        lines.append("const unsigned g = gridFor(t->grid[%d] > 0 ? t->grid[%d] : %d, %s, %d, %s);"
+                     % (pipe.pipeId, pipe.pipeId, pipe.defaultGrid, rows, pipe.bs * pipe.U, per))
     if pa is not None and phase == 0:
         # each workgroup fills its own segment of every partition
         lines[-1] = "const unsigned g = part ? (unsigned)%s : %s;" % (pa["hostNseg"], lines[-1][len("const unsigned g = "):-1])
@@ -749,6 +755,11 @@ def pipelineLaunch(prog, pipe, phase=None, rowsExpr=None, perCu=8):
         # one workgroup per partition; nothing to do without partitions
         lines[-1] = "const unsigned g = (unsigned)%s;" % pa["hostNp"]
         lines.append("if (g > 0u) {")
+    if phase == 1 and getattr(pipe, "globalRefill", False) and pipe.ldsFilter is not None:
+        # the second launch of a kernel-split lane refill runs what follows
+        # the filtered probe: no filter copied into LDS, none allocated
+        lines.append("kp.%s = 0u;" % pipe.ldsFilter["param"])
+        lds = lds.replace(pipe.ldsFilter["host"], "(uint64_t)0")
     kname = pipe.name if phase is None else "%s<%d>" % (pipe.name, phase)
     if pipe.dynLdsExpr is not None:
         # dynamic LDS beyond 64 KB (gfx950 has 160 KB per workgroup) is opted into per kernel
@@ -956,6 +967,12 @@ def useFractions(start, mine, terminal):
         p = t.parent
         a = p.algExpr
         if isinstance(p, (LaneRefill, CrossJoinTranslator)):
+            # rows reach what follows through a refill queue (or per inner
+            # row): left out.  For a refill across launches, loading such a
+            # column at the top of the first launch made that launch stream
+            # every column and left no LDS for a join filter (c3: 0.51 ms for
+            # the first launch alone, against 0.51 ms for the whole kernel
+            # with the LDS queue).
             break
         if isinstance(p, SelectionTranslator):
             use(exprAttrs(a.condition), frac)

@@ -168,6 +168,12 @@ def _featurePlan(name, alg):
                            alg.join(("u_grp", "t_grp"), alg.scan("u"),
                                     alg.selection(scal.SmallerExpr(_A("t_id"), _C("300", Type.INT)), alg.scan("t"))))
         return p, {4: {"multimatch": True}}
+    if name == "f_join_string":
+        # multimatch join on a string key (single-word t_txt = u_name, about
+        # 70 u rows per word), grouped by t_cat
+        p = alg.aggregation(["t_cat"], [(Reduction.COUNT, "", "c"), (Reduction.SUM, "u_w", "s")],
+                            alg.join(("u_name", "t_txt"), alg.scan("u"), alg.scan("t")))
+        return p, {3: {"multimatch": True}}
     if name == "f_semi":
         p = alg.aggregation(["t_cat"], [(Reduction.COUNT, "", "c")],
                             alg.semijoin(("u_grp", "t_grp"), None, alg.scan("u"), alg.scan("t")))
@@ -372,7 +378,7 @@ FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_
                  "f_join_unique_lowest",
                  "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
                  "f_groupby_int64", "f_outer", "f_outer_project", "f_join_neg", "f_anti_neg",
-                 "f_cross", "f_cross_pairs", "f_like_patterns"]
+                 "f_cross", "f_cross_pairs", "f_like_patterns", "f_join_string"]
 
 # name -> (workload or "feature", options)
 PLANS = OrderedDict()
@@ -393,6 +399,10 @@ for _n in FEATURE_PLANS:
 for _n in STRING_PLANS:
     PLANS[_n] = ("strings", {})
 PLANS["f_join_multi_pushdown"] = ("feature", {"pushdown": True, "base": "f_join_multi"})
+PLANS["f_join_string_pushdown"] = ("feature", {"pushdown": True, "base": "f_join_string"})
+# string keys hashed and compared by the whole wavefront (StringHash.wave)
+PLANS["f_join_string_ws"] = ("feature", {"wave_strings": True, "base": "f_join_string"})
+PLANS["f_join_string_pushdown_ws"] = ("feature", {"pushdown": True, "wave_strings": True, "base": "f_join_string"})
 PLANS["f_like_project_refill"] = ("feature", {"buffers": [2], "base": "f_like_project"})
 PLANS["f_like_project_refill_reg"] = ("feature", {"buffers": [2], "base": "f_like_project", "buffer_type": "reg"})
 
@@ -433,6 +443,14 @@ PLANS["tpch10_lr9reg_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": 
 PLANS["tpch3_profiled_reg"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7], "buffer_type": "reg"})
 PLANS["c1_refill_reg"] = ("c1", {"buffers": [4], "buffer_type": "reg"})
 PLANS["c3_refill_reg"] = ("c3", {"buffers": [6], "buffer_type": "reg"})
+# lane refill across a kernel boundary (BufferType.GLOBAL): survivors queued
+# in device memory by a first launch, the rest of the pipeline run over the
+# queue by a second
+PLANS["c1_refill_g"] = ("c1", {"buffers": [4], "buffer_type": "global"})
+PLANS["c2_refill_g"] = ("c2", {"buffers": [3], "buffer_type": "global"})
+PLANS["c3_refill_g"] = ("c3", {"buffers": [6], "buffer_type": "global"})
+PLANS["tpch10_lrg9_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9], "buffer_type": "global"})
+PLANS["tpch3_profiled_g"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7], "buffer_type": "global"})
 
 
 def build(name, acc):
@@ -477,7 +495,7 @@ def emptyAccessor(name):
 
 # plans the autotuner explores compile-time variants of (built ahead of time)
 TUNE_PLANS = ["c0", "c1", "c1_refill", "c2_none", "c2_refill", "c2_pushdown", "c2_both", "c3", "c3_refill",
-              "c1_refill_reg", "c2_refill_reg", "c2_both_reg", "c3_refill_reg"]
+              "c1_refill_reg", "c2_refill_reg", "c2_both_reg", "c3_refill_reg", "c1_refill_g", "c3_refill_g"]
 
 
 def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=None):
@@ -490,7 +508,8 @@ def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=
     with knobs(pushdown=opts.get("pushdown", False), block=block or opts.get("block"),
                threshold=opts.get("threshold"), rowsPerThread=rowsPerThread,
                partBuild=opts.get("part_build", False),
-               bufferType=BufferType.REG if opts.get("buffer_type") == "reg" else BufferType.SMEM,
+               bufferType={"reg": BufferType.REG, "global": BufferType.GLOBAL}.get(opts.get("buffer_type"),
+                                                                                    BufferType.SMEM),
                waveStrings=opts.get("wave_strings", False)):
         comp = HipCompiler(alg, "gfx950", decimalRepr)
         comp.setBuffers(opts.get("buffers", []))

@@ -82,12 +82,12 @@ def test_c0_grid(grid):
     check("c0", workloadAcc("c0", {"cfg0": 300000}, seed=6), tuning=t, label="grid=%d" % grid)
 
 
-@pytest.mark.parametrize("plan", ["c1", "c1_refill", "c1_refill_reg"])
+@pytest.mark.parametrize("plan", ["c1", "c1_refill", "c1_refill_reg", "c1_refill_g"])
 def test_c1_full(plan):
     check(plan, workloadAcc("c1"), storeKey="c1full", label="1.5M x 6M")
 
 
-@pytest.mark.parametrize("plan", ["c1", "c1_refill", "c1_refill_reg"])
+@pytest.mark.parametrize("plan", ["c1", "c1_refill", "c1_refill_reg", "c1_refill_g"])
 @pytest.mark.parametrize("rows", [{"dim": 0, "fact": 1000}, {"dim": 1000, "fact": 0}, {"dim": 1, "fact": 1},
                                   {"dim": 3000, "fact": 20000}])
 def test_c1_small(rows, plan):
@@ -103,7 +103,7 @@ def test_c1_undersized_tables_retry():
     assert retries > 0
 
 
-C2_VARIANTS = ["c2_none", "c2_refill", "c2_pushdown", "c2_both", "c2_refill_reg", "c2_both_reg"]
+C2_VARIANTS = ["c2_none", "c2_refill", "c2_pushdown", "c2_both", "c2_refill_reg", "c2_both_reg", "c2_refill_g"]
 
 
 @pytest.mark.parametrize("variant", C2_VARIANTS)
@@ -205,7 +205,7 @@ def c3Oracle(acc):
     return res, ev.temps["c3_by_nation_year"][0]
 
 
-@pytest.mark.parametrize("plan", ["c3", "c3_refill", "c3_refill_reg"])
+@pytest.mark.parametrize("plan", ["c3", "c3_refill", "c3_refill_reg", "c3_refill_g"])
 @pytest.mark.parametrize("rows", [C3_SMALL, {"sales": 1, "item": 1, "cust": 1, "nation": 1},
                                   {"sales": 0, "item": 100, "cust": 10, "nation": 25},
                                   {"sales": 5000, "item": 0, "cust": 10, "nation": 25},
@@ -232,7 +232,7 @@ def test_c3_temp_overflow_retry():
     assertSameResult(ny, wantNy, {"revenue"}, label="c3 retry by nation, year")
 
 
-@pytest.mark.parametrize("plan", ["c3", "c3_refill_reg"])
+@pytest.mark.parametrize("plan", ["c3", "c3_refill_reg", "c3_refill_g"])
 def test_c3_full_vs_c_oracle(plan):
     from oracle import coracle
     print("gpu case: %s full 60M x 2M x 200k x 25" % plan, flush=True)
@@ -328,7 +328,7 @@ def test_tpch(q, sf):
 
 
 @pytest.mark.parametrize("name", ["tpch10_p9", "tpch10_lr9_p9", "tpch10_lr9reg_p9", "tpch10_ws_lr9_p9",
-                                  "tpch3_profiled", "tpch3_profiled_reg"]
+                                  "tpch10_lrg9_p9", "tpch3_profiled", "tpch3_profiled_reg", "tpch3_profiled_g"]
                          + ["tpch%d_ws" % q for q in planlib.WAVE_STRING_TPCH])
 @pytest.mark.parametrize("sf", [0.05, 0.002])
 def test_tpch_refill_variants(name, sf):
@@ -592,7 +592,7 @@ def filterTuning(kb):
     return t
 
 
-@pytest.mark.parametrize("kb", [-1, 1, 4, 64])
+@pytest.mark.parametrize("kb", [-1, 1, 4, 64, 2048])
 @pytest.mark.parametrize("plan", ["c1", "c1_refill"])
 def test_join_filter_c1(plan, kb):
     """The LDS filter of c1's selective dim join (tuning join_filter): off
@@ -602,7 +602,7 @@ def test_join_filter_c1(plan, kb):
         check(plan, workloadAcc("c1", rows, seed=51), tuning=filterTuning(kb), label="kb=%d %r" % (kb, rows))
 
 
-@pytest.mark.parametrize("kb", [-1, 1, 32, 96])
+@pytest.mark.parametrize("kb", [-1, 1, 32, 96, 1024])
 def test_join_filter_c3(kb):
     print("gpu case: c3_refill join_filter=%d" % kb, flush=True)
     acc = workloadAcc("c3", {"sales": 200000, "item": 20000, "cust": 2000, "nation": 25}, seed=52)
@@ -667,3 +667,26 @@ def test_partitioned_build_c3(parts):
     want, wantNy = c3Oracle(acc)
     assertSameResult(got, want, {"revenue"}, label="c3 by item parts=%d" % parts)
     assertSameResult(ny, wantNy, {"revenue"}, label="c3 by nation, year parts=%d" % parts)
+
+
+@pytest.mark.parametrize("plan", ["c3_refill_g", "c1_refill_g"])
+def test_global_refill_queue_overflow_retry(plan):
+    """A kernel-split refill queue sized far below the survivors overflows:
+    the status reports the count and the run repeats with that capacity."""
+    mod = module(plan)
+    q = [h["index"] for h in mod.desc["hts"] if h["kind"] == "queue"]
+    assert len(q) == 1
+    t = rt.newTuning()
+    t.split_capacity[q[0]] = 16
+    config = "c3" if plan.startswith("c3") else "c1"
+    rows = C3_SMALL if config == "c3" else {"dim": 5000, "fact": 60000}
+    acc = workloadAcc(config, rows, seed=55)
+    if config == "c3":
+        print("gpu case: %s undersized queue" % plan, flush=True)
+        got, ny, retries = c3Run(acc, tuning=t, plan=plan)
+        want, wantNy = c3Oracle(acc)
+        assertSameResult(got, want, {"revenue"}, label="%s retry by item" % plan)
+        assertSameResult(ny, wantNy, {"revenue"}, label="%s retry by nation, year" % plan)
+    else:
+        _, retries = check(plan, acc, tuning=t, label="undersized queue")
+    assert retries > 0
