@@ -10,7 +10,8 @@ relationalAlgebra.py:453-455) plus the MI355X-specific ones:
   run time (dqc_plan_tuning, no recompilation):
     grid size per pipeline, refill threshold, LDS aggregation slots and
     replicas (pipelines annotated LDSn), hash-table size factor, and the
-    size of the LDS join filter together with its probe pipeline's grid
+    size of the join filter (LDS sizes together with the probe pipeline's
+    grid, MB sizes kept in device memory)
 
 Each candidate is timed with HIP events around every launch (median of a few
 runs), so one pipeline's knob is judged by that pipeline's kernel time; the
@@ -181,15 +182,19 @@ class Autotuner(object):
         if bfPipes:
             keep = (tuning.join_filter, [tuning.grid[p] for p in bfPipes])
             bestScore = score(best)
-            for jf in (32, 64, 96, 128):
-                for g in (cus, 2 * cus, 4 * cus):
-                    tuning.join_filter = jf
-                    for p in bfPipes:
-                        tuning.grid[p] = g
-                    times, fits = m.measure(tuning)
-                    self.log.append(("join_filter", jf, g, score(times)))
-                    if fits and score(times) < bestScore * 0.98:
-                        keep, bestScore, best = (jf, [g] * len(bfPipes)), score(times), times
+            tuned = tuning.grid[bfPipes[0]]
+            # LDS sizes with few workgroups; MB sizes stay in device memory
+            # (L2) and keep the pipeline's own grid
+            trials = [(jf, g) for jf in (32, 64, 96, 128) for g in (cus, 2 * cus, 4 * cus)] + \
+                [(jf, tuned) for jf in (1024, 4096)]
+            for jf, g in trials:
+                tuning.join_filter = jf
+                for p in bfPipes:
+                    tuning.grid[p] = g
+                times, fits = m.measure(tuning)
+                self.log.append(("join_filter", jf, g, score(times)))
+                if fits and score(times) < bestScore * 0.98:
+                    keep, bestScore, best = (jf, [g] * len(bfPipes)), score(times), times
             tuning.join_filter = keep[0]
             for p, g in zip(bfPipes, keep[1]):
                 tuning.grid[p] = g

@@ -726,6 +726,33 @@ DQC_DEVICE_INLINE uint64_t string_hash(str_t s) {
     return string_hash_final(sum, n);
 }
 
+// Wavefront-cooperative string equality (the push-down form of the
+// reference's stringEqualsPushdown, util.cuh:104-130): for every active lane
+// whose lengths agree, all 64 lanes compare 8-byte chunks of its two
+// strings (lane l the chunks l, l + 64, ...) and a ballot combines them.
+// Called by every lane of the wavefront (uniform control flow); inactive
+// lanes get false.
+DQC_DEVICE_INLINE bool string_equals_wave(bool active, str_t a, str_t b) {
+    const uint32_t lane = lane_id();
+    bool result = false;
+    uint64_t todo = ballot(active && (a.end - a.start) == (b.end - b.start));
+    while (todo) {
+        const int t = first_lane(todo);
+        todo &= todo - 1u;
+        const str_t x = shfl(a, t);
+        const str_t y = shfl(b, t);
+        const uint64_t n = (uint64_t)(x.end - x.start);
+        bool same = true;
+        for (uint64_t o = 8u * lane; o < n; o += 8u * WAVE) {
+            const uint32_t m = n - o < 8u ? (uint32_t)(n - o) : 8u;
+            same = same && load_bytes(x.start + o, m) == load_bytes(y.start + o, m);
+        }
+        const bool all = ballot(!same) == 0ull;
+        if ((int)lane == t) result = all;
+    }
+    return result;
+}
+
 // Wavefront-cooperative string hash (the push-down form of the reference's
 // stringHashPushDown, hashing.cuh:39-62): the strings of the active lanes
 // are hashed one after another by all 64 lanes, lane l taking the chunks

@@ -61,11 +61,12 @@ class EquiJoinTranslator(BinaryTranslator):
     usePartitionedBuild = False      # unique builds that can be partitioned are translated for it
     SPLIT_CHUNK = 64                 # matches per queued chunk
     DEFAULT_SPLIT_THRESHOLD = 1024   # list length above which a probe row is queued
-    # unique inner joins at most this selective can take an LDS filter of
-    # their build keys (tuning join_filter = n KB; off by default: measured on
-    # c3's item join, 6% selective, the kernel took 598 us without, 794 us
-    # with 32 KB and 1.9 ms with 64 KB -- every workgroup copies the filter
-    # and holds it in LDS, which costs more than the L2 probes it saves)
+    # unique inner joins at most this selective can take a filter of their
+    # build keys (tuning join_filter = n KB, up to 4 MB; off by default): in
+    # LDS when it fits beside the probe pipeline's other LDS (every workgroup
+    # copies it, so it pays with few, large workgroups; the autotuner
+    # searches size and grid together), else probed in device memory, where
+    # a filter of a few MB stays in L2 while the table it guards does not
     FILTER_SELECTIVITY = 0.25
 
     def __init__(self, algExpr, leftChild, rightChild):
@@ -225,7 +226,7 @@ class EquiJoinTranslator(BinaryTranslator):
         if kind == "unique" and self.filtered:
             prog.hostVars["bfw%d" % idx] = (
                 "{ const int32_t req = t->join_filter; uint32_t w = 0u;\n"
-                "  if (req > 0) w = std::min<uint32_t>(32768u, pow2floor((uint64_t)req * 256u));\n"
+                "  if (req > 0) w = std::min<uint32_t>(1u << 20, pow2floor((uint64_t)req * 256u));\n"
                 "  L.bfw[%d] = w; }" % idx)
             bf = prog.buffer("jbf%d" % J, CType.UINT, "std::max<uint32_t>(1u, L.bfw[%d])" % idx, "0x0ull")
             fbf = pipe.param("jbf%d" % J, "uint32_t*", "B_" + bf)
@@ -628,9 +629,13 @@ class EquiJoinTranslator(BinaryTranslator):
                 deps = [indexReg]
             pipe.setLazy(att, load, deps, ct)
 
-    def _matchCond(self, ctxt, pipe, fields, indexReg):
+    def _matchCond(self, ctxt, pipe, fields, indexReg, uniform=False):
         """C expression: stored keys at indexReg[u] equal the probe keys and
-        the extra join conditions hold.  Loads what it reads explicitly."""
+        the extra join conditions hold.  Loads what it reads explicitly.
+        With `uniform` (the caller is at a wavefront-uniform position) and
+        StringHash.wave, string keys are compared by the whole wavefront
+        (emitted here, ahead of the expression)."""
+        from dogqc_amd.hipTranslator import StringHash
         a = self.algExpr
         ix = indexReg + "[u]"
         parts = []
@@ -638,7 +643,14 @@ class EquiJoinTranslator(BinaryTranslator):
             byId = {att.id: f for f, att in zip(fields, self.stored)}
             for b, p in zip(self.buildKeys, self.probeKeys):
                 f = byId[b.id]
-                if b.dataType == Type.STRING:
+                if b.dataType == Type.STRING and uniform and StringHash.wave:
+                    name = "weq%d" % pipe.uniqueId()
+                    # every lane takes part, but only active lanes' positions
+                    # are valid entries: the others pass an empty string
+                    pipe.emit("const bool %s = dqc::string_equals_wave(active[u], active[u] ? %s : dqc::str_t{}, %s);"
+                              % (name, f.at(ix), pipe.access(p)))
+                    parts.append(name)
+                elif b.dataType == Type.STRING:
                     parts.append("dqc::string_equals(%s, %s)" % (f.at(ix), pipe.access(p)))
                 else:
                     parts.append("(%s == %s)" % (f.at(ix), pipe.access(p)))
@@ -680,7 +692,11 @@ class EquiJoinTranslator(BinaryTranslator):
             jw = pipe.reg("jw%d" % J, "dqc::probe_win<%s>" % tt)
             lbf = self._probeFilter(ctxt, pipe, idx)
             jf = pipe.reg("jf%d" % J, "int") if lbf is not None else None
-            passes = "dqc::bloom_test(%s, p.jbfp%d, jh%d)" % (lbf, J, J) if lbf is not None else None
+            # in LDS when it fits (p.jbfpJ words), else read where it lies in
+            # device memory, where it stays in L2 (p.jbfgJ words), else none
+            passes = ("(p.jbfp%d ? dqc::bloom_test(%s, p.jbfp%d, jh%d) : "
+                      "dqc::bloom_test(p.jbf%d, p.jbfg%d, jh%d))" % (J, lbf, J, J, J, J, J)) \
+                if lbf is not None else None
             if not needCheck:
                 # the probe window of every row of the thread first
                 # (independent loads), then the comparisons
@@ -692,7 +708,7 @@ class EquiJoinTranslator(BinaryTranslator):
                     pipe.emit("%s = dqc::probe_start(jh%d, %s);" % (jc, J, fcap))
                     if lbf is not None:
                         # rows the filter rules out load no probe window
-                        pipe.emit("%s = p.jbfp%d == 0u || %s;" % (jf, J, passes))
+                        pipe.emit("%s = (p.jbfp%d == 0u && p.jbfg%d == 0u) || %s;" % (jf, J, J, passes))
                         pipe.emit("if (%s) %s = dqc::probe_window(%s, %s);" % (jf, jw, ftags, jc))
                     else:
                         pipe.emit("%s = dqc::probe_window(%s, %s);" % (jw, ftags, jc))
@@ -705,7 +721,7 @@ class EquiJoinTranslator(BinaryTranslator):
                 if needCheck:
                     self._hashCode(pipe, self.probeKeys, jtag, "jh%d" % J, tagIsReg=True, pre=pre)
                     if lbf is not None:
-                        pipe.emit("%s = p.jbfp%d == 0u || %s;" % (jf, J, passes))
+                        pipe.emit("%s = (p.jbfp%d == 0u && p.jbfg%d == 0u) || %s;" % (jf, J, J, passes))
                         pipe.code.open("if (%s)" % jf)
                     pipe.emit("uint32_t jcur = dqc::probe_start(jh%d, %s), jv = 0;" % (J, fcap))
                     if self.tag32:
@@ -814,12 +830,14 @@ class EquiJoinTranslator(BinaryTranslator):
             % (pid, idx, staticLdsBytes(pipe) + 1024, idx))
         fbf = pipe.param("jbf%d" % J, "const uint32_t*", "B_jbf%d" % J)
         pipe.param("jbfp%d" % J, "uint32_t", "L.bfp[%d]" % pid)
+        pipe.param("jbfg%d" % J, "uint32_t", "L.bfp[%d] ? 0u : L.bfw[%d]" % (pid, idx))
         lbf = "lbf%d" % J
         pipe.prologue.emit("uint32_t* %s = (uint32_t*)dqc_dyn_lds;" % lbf)
         pipe.prologue.emit("for (uint32_t i = threadIdx.x; i < p.jbfp%d; i += %du) %s[i] = %s[i];"
                            % (J, pipe.bs, lbf, fbf))
         pipe.prologue.emit("__syncthreads();")
-        pipe.ldsFilter = {"dev": "(uint64_t)p.jbfp%d * 4u" % J, "host": "(uint64_t)L.bfp[%d] * 4u" % pid}
+        pipe.ldsFilter = {"dev": "(uint64_t)p.jbfp%d * 4u" % J, "host": "(uint64_t)L.bfp[%d] * 4u" % pid,
+                          "param": "jbfp%d" % J}
         if pipe.dynLdsExpr is None:
             pipe.dynLdsExpr = pipe.ldsFilter["host"]
         pipe.annotations.append("BF%d" % J)
@@ -866,7 +884,8 @@ class EquiJoinTranslator(BinaryTranslator):
             pipe.emit("%s = %s;" % (pos, moff))
         self._bindBuild(pipe, fields, posName, nullReg)
         pipe.emit("active[u] = mact%d;" % J)
-        pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, posName))
+        pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, posName,
+                                                                                 uniform=True))
         if outer:
             pipe.emit("if (active[u]) %s = 0;" % opend)
             pipe.emit("if (%s[u]) { active[u] = 1; %s = 0; }" % (nullReg, opend))
@@ -1022,7 +1041,8 @@ class EquiJoinTranslator(BinaryTranslator):
         for r, sv in saved:
             pipe.emit("%s[u] = dqc::shfl(%s, po);" % (r, sv))
         self._bindBuild(pipe, fields, posName)
-        pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, posName))
+        pipe.emit("if (active[u]) active[u] = (%s) ? 1 : 0;" % self._matchCond(ctxt, pipe, fields, posName,
+                                                                                 uniform=True))
         ctxt.innerLoopCount += 1
         pipe.innerDepth += 1
         self.parent.consume(ctxt)
