@@ -216,7 +216,10 @@ def main():
     dom_ms = kernel_ms[dom] if dom in kernel_ms else sum(v for k, v in kernel_ms.items() if k.startswith(dom + "<"))
     alg_bytes = spec["bytes_per_row"] * probe_rows
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = loadTraffic(args.workload, dom)
+    # traffic summaries of other sizes than the workload's own carry the
+    # probe rows in their name (c0 at 64M rows: krnl_cfg01_c0_64M_traffic.json)
+    tag = args.workload if args.rows_scale == 1.0 else "%s_%dM" % (args.workload, probe_rows // 1000000)
+    traffic = loadTraffic(tag, dom)
 
     out = {
         "metric": METRIC,
@@ -237,7 +240,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": ("profiles/%s_%s_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                                         "of this kernel, copy-kernel calibrated; read from the committed file, "
-                                        "not measured in this run" % (dom, args.workload)) if traffic else None,
+                                        "not measured in this run" % (dom, tag)) if traffic else None,
                      "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
         "kernels_ms": kernel_ms,

@@ -13,6 +13,10 @@ relationalAlgebra.py:453-455) plus the MI355X-specific ones:
     size of the join filter (LDS sizes together with the probe pipeline's
     grid, MB sizes kept in device memory)
 
+A plan whose estimates are too small for the inputs (a fixed group count, a
+selectivity) is first run with retries, and the capacities the retries grew
+to become part of the search's starting point and of the record.
+
 Each candidate is timed with HIP events around every launch (median of a few
 runs), so one pipeline's knob is judged by that pipeline's kernel time; the
 table size factor is judged by the whole plan (it moves cost between the
@@ -39,6 +43,10 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CACHE_DIR = os.path.join(PKG, "_tuning")
 
 VARIANTS = [(256, 4), (256, 2), (256, 1), (512, 4), (128, 4), (256, 8), (512, 1), (1024, 1)]
+
+
+# dqc_plan_tuning fields a retry grows (dqc_runtime.hip adjustForRetry)
+CAPACITY_FIELDS = ("ht_capacity", "mm_entries", "result_capacity", "temp_capacity", "split_capacity")
 
 
 def variantName(name, block, rows):
@@ -96,6 +104,9 @@ class Measurer(object):
         trial = tuningFromDict(tuningToDict(tuning))  # retries must not leak into `tuning`
         ex = rt.run(self.module, self.inputs, trial, self.stream, max_retries=6)
         ok = ex.res.retries == 0
+        # the table, temp and queue capacities the retries grew to
+        self.grown = {f: list(getattr(trial, f)) if hasattr(getattr(trial, f), "__len__") else getattr(trial, f)
+                      for f in CAPACITY_FIELDS}
         need = ex.res.layout.ws_bytes
         ex.release()
         if self.ws is None or self.ws.nbytes < need:
@@ -129,6 +140,19 @@ class Autotuner(object):
         cus = rt.device()[0]
         m = Measurer(module, inputs, self.reps)
         best, ok = m.measure(tuning)
+        if not ok:
+            # the plan's estimates are too small for these inputs (a fixed
+            # group count, a selectivity): start from the capacities its
+            # retries reached, so that every candidate runs without a retry
+            # and the record carries them for queries of this size
+            for f, v in m.grown.items():
+                if isinstance(v, list):
+                    arr = getattr(tuning, f)
+                    for i, x in enumerate(v):
+                        arr[i] = x
+                else:
+                    setattr(tuning, f, v)
+            best, ok = m.measure(tuning)
         steps = m.steps
 
         def kernelOf(pipe):
@@ -167,6 +191,8 @@ class Autotuner(object):
                 if tuning.lds_slots[pid] >= 0:
                     trySet("lds_replicas", pid, [1, 4, 16, 64], pipe)
         for ht in module.desc["hts"]:
+            if tuning.ht_capacity[ht["index"]] > 0:
+                continue        # an explicit capacity overrides the factor
             if ht["kind"] in ("unique", "agg"):
                 trySet("ht_factor", ht["index"], [0.5, 0.75, 1.5, 2.0, 4.0], None)
             elif ht["kind"] == "multi":
@@ -186,7 +212,7 @@ class Autotuner(object):
             # LDS sizes with few workgroups; MB sizes stay in device memory
             # (L2) and keep the pipeline's own grid
             trials = [(jf, g) for jf in (32, 64, 96, 128) for g in (cus, 2 * cus, 4 * cus)] + \
-                [(jf, tuned) for jf in (1024, 4096)]
+                [(jf, tuned) for jf in (1024, 2048, 4096)]
             for jf, g in trials:
                 tuning.join_filter = jf
                 for p in bfPipes:

@@ -453,6 +453,12 @@ class Program(object):
                 expr = "(int64_t)%d" % int(node.numgroups)
             else:
                 expr = self.estimate(node.child)
+                # keys of small domains (CHAR, BOOLEAN) bound the group count
+                # whatever the input size: the table and the LDS tier are
+                # sized for that many groups, not for the input rows
+                dom = _keyDomain(node.groupAttributes.values())
+                if dom is not None:
+                    expr = "std::min<int64_t>(%s, (int64_t)%d)" % (expr, dom)
         else:
             raise NotImplementedError(type(node).__name__)
         self.estimates[node.opId] = expr
@@ -718,6 +724,26 @@ def _asCode(text):
 
 def _numgroupsFromChild(node):
     return not getattr(node, "numgroupsFromCfg", False)
+
+
+# distinct values a group key of this type can take, for the types whose
+# domain is small enough to bound a table
+_KEY_DOMAIN = {Type.CHAR: 256, Type.BOOLEAN: 2}
+
+
+def _keyDomain(attrs):
+    """Number of distinct key tuples the group attributes can form, or None
+    when one of them has no small domain or the product exceeds 2^16."""
+    attrs = list(attrs)
+    if not attrs:
+        return None
+    d = 1
+    for a in attrs:
+        n = _KEY_DOMAIN.get(getattr(a, "dataType", None))
+        if n is None:
+            return None
+        d *= n
+    return d if d <= (1 << 16) else None
 
 
 def pipelineLaunch(prog, pipe, phase=None, rowsExpr=None, perCu=8):
