@@ -333,7 +333,7 @@ def saveRecord(rec, directory=None):
     return path
 
 
-def tuneServing(plans, scales=(0.1, 1.0, 10.0), tableScale=10.0, reps=5, out=None, log=print):
+def tuneServing(plans, scales=(0.1, 1.0, 10.0), tableScale=10.0, reps=5, out=None, log=print, variants=None):
     """Tune plans on HBM-resident tables at `tableScale` x the base sizes
     (generated on the device), once per row scale: a query at scale s reads
     the first s x base rows of every table, as the serving workload's do."""
@@ -345,7 +345,7 @@ def tuneServing(plans, scales=(0.1, 1.0, 10.0), tableScale=10.0, reps=5, out=Non
         config = planlib.PLANS[name][0]
         for sc in scales:
             rows = serving.prefixRows(tables, config, sc)
-            tuner = Autotuner(name, acc, reps=reps, verbose=True)
+            tuner = Autotuner(name, acc, variants=variants, reps=reps, verbose=True)
             best, allRecs = tuner.run(store=tables.store, rows=rows)
             best["scale"] = sc
             path = saveRecord(best, out)
@@ -370,16 +370,24 @@ def main(argv=None):
                     help="tune at 0.1x, 1x and 10x on device-resident tables (records to --out-dir)")
     ap.add_argument("--scales", default="0.1,1,10")
     ap.add_argument("--out-dir", default=None, help="write records here instead of the package's record folder")
+    ap.add_argument("--variants", default=None,
+                    help="compile-time variants to search, as BLOCKxROWS[,...] (default: all eight)")
     args = ap.parse_args(argv)
+    variants = None
+    if args.variants:
+        variants = [tuple(int(x) for x in v.split("x")) for v in args.variants.split(",")]
+        for v in variants:
+            if v not in VARIANTS:
+                ap.error("variant %dx%d is not built (built: %s)" % (v + (", ".join("%dx%d" % w for w in VARIANTS),)))
     if args.serving:
         tuneServing(args.plan, [float(x) for x in args.scales.split(",")], reps=args.reps, out=args.out_dir,
-                    log=lambda m: print(m, flush=True))
+                    log=lambda m: print(m, flush=True), variants=variants)
         return
     args.plan = args.plan[0]
     kind = planlib.PLANS[args.plan][0]
     rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[kind].items()}
     acc = MemoryAccessor(workloads.generate(kind, rows))
-    tuner = Autotuner(args.plan, acc, reps=args.reps, verbose=True)
+    tuner = Autotuner(args.plan, acc, variants=variants, reps=args.reps, verbose=True)
     best, allRecs = tuner.run()
     best["scale"] = args.rows_scale
     if args.out_dir:
