@@ -306,6 +306,58 @@ DQC_DEVICE_INLINE int64_t probe_in_window(const probe_win<Tag>& w, uint32_t s, u
     return r;
 }
 
+// Packed unique tables: a key packing into 32 bits with one stored column of
+// at most 4 bytes keeps both in one 64-bit slot, the tag (packed key + 1) in
+// the low half and the column's bits in the high half, so a probe that finds
+// its key has the column in the line its window load brought (one cache miss
+// per probe instead of one for the tag and one for the column).  0 is the
+// empty slot; the tag-0 key lives in the extra slot at index cap with tag 1.
+DQC_DEVICE_INLINE uint64_t packed_slot(uint32_t tag, uint64_t word) {
+    return (word << 32) | (uint64_t)(tag != 0u ? tag : 1u);
+}
+
+template <typename T>
+DQC_DEVICE_INLINE T packed_payload(uint64_t slot) {
+    T v;
+    const uint32_t hi = (uint32_t)(slot >> 32);
+    __builtin_memcpy(&v, &hi, sizeof(T));
+    return v;
+}
+
+// probe_in_window over packed slots: the low halves hold the tags
+DQC_DEVICE_INLINE int64_t probe_in_window_packed(const probe_win<uint64_t>& w, uint32_t s, uint32_t cap,
+                                                 uint32_t tag, uint32_t& cursor, uint32_t& visited) {
+    int64_t r = -2;
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < PROBE_WINDOW; ++i) {
+        if (r == -2 && s + i < cap) {
+            ++n;
+            if ((uint32_t)w.t[i] == tag)
+                r = (int64_t)(s + i);
+            else if (w.t[i] == 0ull)
+                r = -1;
+        }
+    }
+    visited = n;
+    cursor = s + n >= cap ? 0u : s + n;
+    return r;
+}
+
+// ht_probe over packed slots
+DQC_DEVICE_INLINE int64_t ht_probe_packed(const uint64_t* slots, uint32_t cap, uint32_t tag, uint32_t& cursor,
+                                          uint32_t& visited) {
+    while (visited < cap && visited < MAX_PROBE) {
+        const uint32_t s = cursor;
+        const uint64_t t = slots[s];
+        cursor = next_slot(s, cap);
+        ++visited;
+        if ((uint32_t)t == tag) return (int64_t)s;
+        if (t == 0ull) { visited = cap; return -1; }
+    }
+    return -1;
+}
+
 // Group slot in exact mode: the tag is the key, so find-or-claim is one CAS
 // per probed slot with no lock.  Starts at slot s with n slots tried.
 // Returns -1 when the table is full.

@@ -55,10 +55,24 @@ class StoredField(object):
         return "%s[%s]" % (self.name, ix) if self.k is None else "%s[%s].f%d" % (self.name, ix, self.k)
 
 
+class PackedField(object):
+    """The stored column of a packed unique table (one stored column of at
+    most 4 bytes, keys of at most 32 bits): the high half of slot ``ix``."""
+
+    def __init__(self, name, ctype):
+        self.name, self.ctype = name, ctype
+
+    def at(self, ix):
+        return "dqc::packed_payload<%s>(%s[%s])" % (self.ctype, self.name, ix)
+
+
 class EquiJoinTranslator(BinaryTranslator):
 
     usePushDownJoin = False
     usePartitionedBuild = False      # unique builds that can be partitioned are translated for it
+    # unique tables with 32-bit tags and one stored column of at most 4 bytes
+    # keep tag and column in one 64-bit slot (dqc::packed_slot)
+    usePackedSlots = True
     SPLIT_CHUNK = 64                 # matches per queued chunk
     DEFAULT_SPLIT_THRESHOLD = 1024   # list length above which a probe row is queued
     # unique inner joins at most this selective can take a filter of their
@@ -114,6 +128,10 @@ class EquiJoinTranslator(BinaryTranslator):
         self.tag32 = (self.exact and not a.multimatch and
                       sum(PACK_BITS[b.dataType] for b in self.buildKeys) <= 32)
         self.tagType = "uint32_t" if self.tag32 else "uint64_t"
+        # the slot word of a unique table: the tag, or tag and stored column
+        # (set by the build, which is translated before the probe)
+        self.packed = False
+        self.slotType = self.tagType
         # a selective unique join gets an LDS filter of its build keys
         self.filtered = (not a.multimatch and a.joinType == Join.INNER
                          and float(a.selectivity) <= self.FILTER_SELECTIVITY)
@@ -185,11 +203,14 @@ class EquiJoinTranslator(BinaryTranslator):
         self.storedBuffers = []
         if kind == "unique":
             prog.hostVars["mment%d" % idx] = "L.mment[%d] = 0;" % idx
+            if self._packable(pipe):
+                self.packed, self.slotType = True, "uint64_t"
+                self.packedType = pipe.attrs[self.stored[0].id].ctype
             # padded by PROBE_WINDOW slots for the probe's window loads (with
             # 32-bit tags the first of them is the extra slot)
-            tt = CType.UINT if self.tag32 else CType.UINT64
+            tt = CType.UINT if self.slotType == "uint32_t" else CType.UINT64
             tags = prog.buffer("jht%d_tags" % J, tt, "L.htcap[%d] + 4" % idx, "0x0ull")
-            ftags = pipe.param("jht%d" % J, "%s*" % self.tagType, "B_" + tags)
+            ftags = pipe.param("jht%d" % J, "%s*" % self.slotType, "B_" + tags)
             fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
             count = "L.htcap[%d] + 1" % idx
         else:
@@ -204,7 +225,9 @@ class EquiJoinTranslator(BinaryTranslator):
             fecap = pipe.param("jht%d_ecap" % J, "uint32_t", "L.mment[%d]" % idx)
             count = "L.mment[%d]" % idx
         fields = []
-        if kind == "unique":
+        if kind == "unique" and self.packed:
+            fields = [PackedField(ftags, self.packedType)]
+        elif kind == "unique":
             for k, att in enumerate(self.stored):
                 ct = pipe.attrs[att.id].ctype
                 b = prog.buffer("jht%d_p%d" % (J, k), ct, count)
@@ -234,7 +257,7 @@ class EquiJoinTranslator(BinaryTranslator):
         if kind == "unique":
             tt = self.tagType
             ins = {"jtag": pipe.reg("jtag%d" % J, tt), "jc": pipe.reg("jc%d" % J, "uint32_t"),
-                   "jprev": pipe.reg("jprev%d" % J, tt), "ftags": ftags, "fcap": fcap, "fields": fields,
+                   "jprev": pipe.reg("jprev%d" % J, self.slotType), "ftags": ftags, "fcap": fcap, "fields": fields,
                    "bloom": (fbf, fbfw) if fbf is not None else None}
             if self._buildPartitionable(pipe):
                 self._partitionedBuild(ctxt, pipe, prog, ins, est)
@@ -262,11 +285,32 @@ class EquiJoinTranslator(BinaryTranslator):
             pipe.code.close()
             pipe.code.close()
 
+    def _packable(self, pipe):
+        """Tag and stored column fit one 64-bit slot: 32-bit tags, one
+        stored column of at most 4 bytes, and a build that is not translated
+        for the partitioned insert (which writes tags and columns apart)."""
+        if not (EquiJoinTranslator.usePackedSlots and self.tag32 and len(self.stored) == 1):
+            return False
+        ct = pipe.attrs[self.stored[0].id].ctype
+        if ct == CType.STR_TYPE or CType.size.get(ct, 16) > 4:
+            return False
+        return not self._buildPartitionable(pipe)
+
+    def _probeFn(self):
+        return "dqc::ht_probe_packed" if self.packed else "dqc::ht_probe<%s>" % self.tagType
+
+    def _packedWord(self, pipe, jtag):
+        return "dqc::packed_slot(%s, dqc::to_word<%s>(%s))" % (jtag, self.packedType, pipe.access(self.stored[0]))
+
     def _firstCas(self, pipe, ins):
         """Home slot and first CAS of a row whose tag and hash jhJ are set."""
         J = self.algExpr.opId
         jtag, jc, jprev, ftags, fcap, tt = ins["jtag"], ins["jc"], ins["jprev"], ins["ftags"], ins["fcap"], self.tagType
-        if self.tag32:
+        if self.packed:
+            # tag 0: the extra slot at index cap, with tag 1 in the low half
+            pipe.emit("%s = %s != 0u ? dqc::slot_of(jh%d, %s) : %s;" % (jc, jtag, J, fcap, fcap))
+            pipe.emit("%s = dqc::cas_tag<uint64_t>(&%s[%s], 0ull, %s);" % (jprev, ftags, jc, self._packedWord(pipe, jtag)))
+        elif self.tag32:
             # tag 0: the extra slot at index cap (duplicates of that key are dropped)
             pipe.emit("%s = %s != 0u ? dqc::slot_of(jh%d, %s) : %s;" % (jc, jtag, J, fcap, fcap))
             pipe.emit("%s = dqc::cas_tag(&%s[%s], (%s)0, %s != 0u ? %s : 1u);" % (jprev, ftags, jc, tt, jtag, jtag))
@@ -280,19 +324,24 @@ class EquiJoinTranslator(BinaryTranslator):
         jtag, jc, jprev, ftags, fcap = ins["jtag"], ins["jc"], ins["jprev"], ins["ftags"], ins["fcap"]
         with pipe.rowLoop():
             pipe.code.open("if (%s)" % cond)
-            if self.tag32:
+            if self.packed:
+                pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : %s == 0u ? (int64_t)-2 : "
+                          "dqc::ht_insert_from<uint64_t>(%s, %s, %s, dqc::next_slot(%s, %s), 1u, &st->flags);"
+                          % (jprev, jc, jtag, ftags, fcap, self._packedWord(pipe, jtag), jc, fcap))
+            elif self.tag32:
                 pipe.emit("const int64_t js = %s == 0u ? (int64_t)%s : %s == 0u ? (int64_t)-2 : "
                           "dqc::ht_insert_from<uint32_t>(%s, %s, %s, dqc::next_slot(%s, %s), 1u, &st->flags);"
                           % (jprev, jc, jtag, ftags, fcap, jtag, jc, fcap))
             else:
                 pipe.emit("const int64_t js = %s == 0ull ? (int64_t)%s : dqc::ht_insert_from<uint64_t>(%s, %s, %s, "
                           "dqc::next_slot(%s, %s), 1u, &st->flags);" % (jprev, jc, ftags, fcap, jtag, jc, fcap))
-            pipe.code.open("if (js >= 0)")
-            for f, v in zip(ins["fields"], values):
-                pipe.emit("%s = %s;" % (f.at("js"), v))
-            pipe.code.close()
-            pipe.emit("else if (js == -1) { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
-                      "atomicOr(&st->ht_overflow_mask, %du); }" % (1 << idx))
+            if not self.packed:
+                pipe.code.open("if (js >= 0)")
+                for f, v in zip(ins["fields"], values):
+                    pipe.emit("%s = %s;" % (f.at("js"), v))
+                pipe.code.close()
+            pipe.emit("%sif (js == -1) { dqc::raise_status(&st->flags, DQC_ST_HT_OVERFLOW); "
+                      "atomicOr(&st->ht_overflow_mask, %du); }" % ("" if self.packed else "else ", 1 << idx))
             pipe.code.close()
 
     def _directInsert(self, pipe, ins, cond):
@@ -606,13 +655,17 @@ class EquiJoinTranslator(BinaryTranslator):
         if self.ht["kind"] != "unique":
             fe = pipe.param("jht%d_e" % J, "const %s*" % self.entryRecord, "B_jht%d_e" % J)
             return [StoredField(fe, k) for k in range(len(self.stored))]
+        if self.packed:
+            return [PackedField(pipe.param("jht%d" % J, "const uint64_t*", "B_jht%d_tags" % J), self.packedType)]
         out = []
         for k, (b, ct) in enumerate(self.storedBuffers):
             out.append(StoredField(pipe.param("jht%d_p%d" % (J, k), "const %s*" % ct, "B_" + b)))
         return out
 
     def _storedType(self, k):
-        return self.storedBuffers[k][1] if self.ht["kind"] == "unique" else self.storedTypes[k]
+        if self.ht["kind"] != "unique":
+            return self.storedTypes[k]
+        return self.packedType if self.packed else self.storedBuffers[k][1]
 
     def _bindBuild(self, pipe, fields, indexReg, nullReg=None):
         """Build-side attributes become lazy loads from the stored columns at
@@ -679,7 +732,8 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.need(condProbe)
         if self.ht["kind"] == "unique":
             tt = self.tagType
-            ftags = pipe.param("jht%d" % J, "const %s*" % tt, "B_jht%d_tags" % J)
+            wt = self.slotType      # the slot word: uint64_t for packed slots
+            ftags = pipe.param("jht%d" % J, "const %s*" % wt, "B_jht%d_tags" % J)
             fcap = pipe.param("jht%d_cap" % J, "uint32_t", "L.htcap[%d]" % idx)
             fields = self._storedFields(pipe)
             slotName = "jslot%d" % J
@@ -689,7 +743,7 @@ class EquiJoinTranslator(BinaryTranslator):
             needCheck = (not self.exact) or a.conditions is not None
             jtag = pipe.reg("jtag%d" % J, tt)
             jc = pipe.reg("jc%d" % J, "uint32_t")
-            jw = pipe.reg("jw%d" % J, "dqc::probe_win<%s>" % tt)
+            jw = pipe.reg("jw%d" % J, "dqc::probe_win<%s>" % wt)
             lbf = self._probeFilter(ctxt, pipe, idx)
             jf = pipe.reg("jf%d" % J, "int") if lbf is not None else None
             # in LDS when it fits (p.jbfpJ words), else read where it lies in
@@ -734,7 +788,7 @@ class EquiJoinTranslator(BinaryTranslator):
                         pipe.code.open("else for (;;)")
                     else:
                         pipe.code.open("for (;;)")
-                    pipe.emit("%s = dqc::ht_probe<%s>(%s, %s, %s, jcur, jv);" % (slot, tt, ftags, fcap, jtag))
+                    pipe.emit("%s = %s(%s, %s, %s, jcur, jv);" % (slot, self._probeFn(), ftags, fcap, jtag))
                     pipe.emit("if (%s < 0) break;" % slot)
                     pipe.emit("if (%s) break;" % self._matchCond(ctxt, pipe, fields, slotName))
                     pipe.code.close()
@@ -744,10 +798,11 @@ class EquiJoinTranslator(BinaryTranslator):
                     if lbf is not None:
                         pipe.code.open("if (%s)" % jf)
                     pipe.emit("uint32_t jcur = 0, jv = 0;")
-                    pipe.emit("%s = dqc::probe_in_window<%s>(%s, %s, %s, %s, jcur, jv);"
-                              % (slot, tt, jw, jc, fcap, jtag))
-                    pipe.emit("if (%s == -2) %s = dqc::ht_probe<%s>(%s, %s, %s, jcur, jv);"
-                              % (slot, slot, tt, ftags, fcap, jtag))
+                    pipe.emit("%s = %s(%s, %s, %s, %s, jcur, jv);"
+                              % (slot, "dqc::probe_in_window_packed" if self.packed else
+                                 "dqc::probe_in_window<%s>" % tt, jw, jc, fcap, jtag))
+                    pipe.emit("if (%s == -2) %s = %s(%s, %s, %s, jcur, jv);"
+                              % (slot, slot, self._probeFn(), ftags, fcap, jtag))
                     if self.tag32:
                         pipe.emit("if (%s == 0u) %s = %s[%s] != 0u ? (int64_t)%s : -1;"
                                   % (jtag, slot, ftags, fcap, fcap))
@@ -763,7 +818,7 @@ class EquiJoinTranslator(BinaryTranslator):
                         p = self.probeKeys[self.buildKeys.index(b)]
                         st = pipe.setComputed(b)
                         pipe.emit("if (active[u]) %s[u] = %s;" % (st.reg, pipe.access(p)))
-            if a.joinType == Join.INNER and self.exact:
+            if a.joinType == Join.INNER and self.exact and not self.packed:
                 # a group-by on this join's key can accumulate at the join slot
                 pipe.probedJoins.append({"join": self, "slot": slotName, "htIndex": idx, "opId": J,
                                          "loopOps": pipe.loopOps,
