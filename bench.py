@@ -238,9 +238,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": ("profiles/%s_%s_traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
-                                        "of this kernel, copy-kernel calibrated; read from the committed file, "
-                                        "not measured in this run" % (dom, tag)) if traffic else None,
+                     "traffic_source": ("profiles/%s_%s_traffic.json: rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B and "
+                                        "TCC_EA0_WRREQ_WRITE_DRAM_32B of this kernel (L2-to-memory bytes in 32-byte "
+                                        "units); read from the committed file, not measured in this run"
+                                        % (dom, tag)) if traffic else None,
                      "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
         "kernels_ms": kernel_ms,
