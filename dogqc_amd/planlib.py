@@ -206,6 +206,16 @@ def _featurePlan(name, alg):
                                                                 _C("1", Type.INT)), alg.scan("u")),
                                      alg.scan("t")))
         return p, {4: {"multimatch": False}}
+    if name == "f_join_packed":
+        # unique join on a key that includes -1 (the extra slot) carrying one
+        # 4-byte build column: tag and column share one 64-bit slot
+        p = alg.aggregation(["t_cat"], [(Reduction.SUM, "u_grp", "s"), (Reduction.COUNT, "", "c"),
+                                        (Reduction.MIN, "u_grp", "lo"), (Reduction.MAX, "u_grp", "hi")],
+                            alg.join(("uk", "t_grp"),
+                                     alg.map("uk", scal.SubExpr(scal.DivExpr(_A("u_id"), _C("3", Type.INT)),
+                                                                _C("1", Type.INT)), alg.scan("u")),
+                                     alg.scan("t")))
+        return p, {4: {"multimatch": False}}
     if name == "f_anti_neg":
         p = alg.aggregation(["t_cat"], [(Reduction.COUNT, "", "c")],
                             alg.antijoin(("uk", "t_grp"), None,
@@ -379,7 +389,7 @@ FEATURE_PLANS = ["f_scan_minmax", "f_groupby_hashed", "f_groupby_wide", "f_like_
                  "f_join_unique_lowest",
                  "f_groupjoin", "f_join_multi", "f_semi", "f_anti_multi", "f_temp", "f_temp_strings", "f_groupby_64",
                  "f_groupby_int64", "f_outer", "f_outer_project", "f_join_neg", "f_anti_neg",
-                 "f_cross", "f_cross_pairs", "f_like_patterns", "f_join_string"]
+                 "f_join_packed", "f_cross", "f_cross_pairs", "f_like_patterns", "f_join_string"]
 
 # name -> (workload or "feature", options)
 PLANS = OrderedDict()
@@ -419,7 +429,7 @@ for _q in range(1, 23):
     PLANS["tpch%d" % _q] = ("tpch", {"query": _q})
 
 # plans translated with the partitioned unique build (tuning part_build)
-PB_FEATURES = ["f_groupjoin", "f_join_unique_lowest", "f_semi", "f_temp", "f_join_neg", "f_anti_neg"]
+PB_FEATURES = ["f_groupjoin", "f_join_unique_lowest", "f_semi", "f_temp", "f_join_neg", "f_anti_neg", "f_join_packed"]
 PB_TPCH = [3, 5, 9, 18, 21]
 for _n in PB_FEATURES:
     PLANS[_n + "_pb"] = ("feature", {"part_build": True, "base": _n})
