@@ -396,7 +396,7 @@ def benchServing(args):
                                              "batched": r["batched_at_rate_by_config"]},
         "batched_at_high_rate": r["batched_at_high_rate"],
         "batched_at_high_rate_by_config": r["batched_at_high_rate_by_config"],
-        "north_star_check": {
+        "serving_target_check": {
             "batched_offered_qps": r["batched_at_high_rate"]["offered_qps"],
             "batched_x_serial_capacity": r["batched_at_high_rate"]["offered_x_serial_capacity"],
             "batched_p99_ms": r["batched_at_high_rate"]["p99_ms"],
