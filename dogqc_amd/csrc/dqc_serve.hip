@@ -157,6 +157,11 @@ struct Server {
     std::unique_ptr<Arena> arena;
     std::vector<Slot> slots;
     dqc_plan_status* pinned = nullptr;   // one status per slot
+    // per slot: a batch's prefix row counts staged in pinned host memory, so
+    // their copy to the device is asynchronous (a copy from pageable memory
+    // is not); a slot's stage is reused only after its query completed
+    static constexpr size_t PREFIX_STAGE_WORDS = 512;
+    int64_t* prefixPinned = nullptr;
 
     std::mutex mu;
     std::condition_variable cv;
@@ -238,6 +243,7 @@ struct Server {
             b->prefixCols.push_back((int32_t)c);
             for (int64_t id : group) b->prefix.push_back(reqs[id]->rows[m0.batchTableMap[j]]);
         }
+        if (b->prefix.size() > PREFIX_STAGE_WORDS) return -1;   // more than a slot's stage holds: no batch
         b->inputs.ntables = (int32_t)nt;
         b->inputs.ncols = (int32_t)nc;
         b->bindInputs();
@@ -364,7 +370,9 @@ struct Server {
             const size_t N = r.members.size();
             for (size_t i = 0; i < r.prefixCols.size(); ++i) r.cols[r.prefixCols[i]] = dev + i * N;
             r.bindInputs();
-            e = hipMemcpyAsync(dev, r.prefix.data(), prefixBytes, hipMemcpyHostToDevice, s.stream);
+            int64_t* stage = prefixPinned + (size_t)si * PREFIX_STAGE_WORDS;
+            std::memcpy(stage, r.prefix.data(), prefixBytes);
+            e = hipMemcpyAsync(dev, stage, prefixBytes, hipMemcpyHostToDevice, s.stream);
         }
         rc = e == hipSuccess ? r.module->enqueue(&r.inputs, &r.tuning, r.rec.ws, L.ws_bytes, (void*)s.stream, nullptr)
                              : DQC_EHIP;
@@ -565,6 +573,13 @@ int dqc_serve_create(const dqc_serve_config* cfg, void** server) {
         (void)hipFree(base);
         return dqcrt::hipFail(e, "hipHostMalloc");
     }
+    e = hipHostMalloc((void**)&s->prefixPinned, sizeof(int64_t) * Server::PREFIX_STAGE_WORDS * cfg->slots,
+                      hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipHostFree(s->pinned);
+        (void)hipFree(base);
+        return dqcrt::hipFail(e, "hipHostMalloc");
+    }
     s->slots.resize(cfg->slots);
     for (Slot& sl : s->slots) {
         HIP_CHECK(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -686,6 +701,7 @@ int dqc_serve_destroy(void* server) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     if (s->pinned) (void)hipHostFree(s->pinned);
+    if (s->prefixPinned) (void)hipHostFree(s->prefixPinned);
     if (s->arenaBase) (void)hipFree(s->arenaBase);
     delete s;
     return DQC_OK;

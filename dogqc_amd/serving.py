@@ -366,9 +366,19 @@ def benchmark(scale=10.0, nqueries=200, slots=16, arena_gb=48.0, load=0.7, seed=
     say("resident tables: %.1f GB in %.1f s" % (tables.nbytes() / 1e9, time.perf_counter() - t0))
     plans = PlanSet()
     arena = int(arena_gb * (1 << 30))
-    # warm up every module once (code objects load at their first launch)
-    run(tables, plans, [Query(c, 0.1) for c in CONFIG_PLANS], 1, arena, log=say)[1].destroy()
-    say("modules loaded and warmed up")
+    # warm up every module once (code objects load at their first launch):
+    # the tuned variant of each size bucket (queries at 0.1x, 1x and 10x),
+    # and the batched plans (four equal queries per config wait together, so
+    # each config's batched plan runs once)
+    run(tables, plans, [Query(c, s) for c in CONFIG_PLANS for s in (0.1, 1.0, 10.0)], 1, arena,
+        log=say)[1].destroy()
+    hvWarm = {"max_heavy": max_heavy, "heavy_bytes": int(heavy_mb * (1 << 20)), "max_batch": max(max_batch, 4),
+              "batch_ratio": batch_ratio, "batch_min_share": batch_min_share}
+    warm = [Query(c, 0.1) for c in CONFIG_PLANS for _ in range(4)]
+    recsW, srvW = run(tables, plans, warm, slots, arena, log=say, **hvWarm)
+    srvW.destroy()
+    say("modules loaded and warmed up (%d of %d warm-up queries in batches)"
+        % (sum(1 for r in recsW if r.batch_size > 1), len(warm)))
     queries = makeMix(nqueries, seed)
     # the verified sample: the smallest queries of each config, so the CPU
     # check over host copies of their prefixes stays within seconds
