@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c1_refill", choices=sorted(WORKLOADS) + ["c4", "q10"])
+    ap.add_argument("--workload", default="c1_refill", choices=sorted(WORKLOADS) + ["c4", "q10", "tpch"])
     ap.add_argument("--sf", type=float, default=1.0, help="q10: TPC-H scale factor of the synthetic data")
     ap.add_argument("--scale", type=float, default=10.0, help="c4: resident table scale")
     ap.add_argument("--slots", type=int, default=16, help="c4: concurrent queries")
@@ -143,6 +143,8 @@ def main():
         return benchServing(args)
     if args.workload == "q10":
         return benchQ10(args)
+    if args.workload == "tpch":
+        return benchTpch(args)
     import numpy as np
     from dogqc_amd import planlib, workloads, runtime as rt, runner
     from dogqc_amd.dbio import MemoryAccessor
@@ -432,6 +434,101 @@ def graphReplay(mod, inputs, tuning, ws, stream, steps):
     raw = ws.toArray(np.uint8, C.sizeof(rt.Status))
     status = rt.Status.from_buffer_copy(raw.tobytes())
     return (t1 - t0) * 1e3 / steps, status
+
+
+def benchTpch(args):
+    """TPC-H Q1-Q22 on synthetic data at --sf, each plan as `dogqc_amd.tpch`
+    writes it (untuned): per query the median time of --steps enqueues
+    (after --warmup), its kernels, retries of the first run, and its result
+    against the numpy plan oracle (whose time is the CPU column)."""
+    from dogqc_amd import planlib, runtime as rt, runner, tpch
+    from dogqc_amd.dbio import MemoryAccessor
+    from oracle.planeval import evaluate
+    t0 = time.perf_counter()
+    acc = MemoryAccessor(tpch.generate(args.sf, seed=10))
+    gen_s = time.perf_counter() - t0
+    store = {}
+    stream = rt.Stream()
+    queries = {}
+    for q in range(1, 23):
+        name = "tpch%d" % q
+        mod = rt.PlanModule(planlib.translate(name).compile(name))
+        inputs = rt.BoundInputs(mod, acc, store=store)
+        tun = rt.newTuning()
+        ex = rt.run(mod, inputs, tun, stream)       # retries grow tun's capacities in place
+        retries = int(ex.res.retries)
+        got = runner.resultColumns(mod, ex, acc)
+        lay = ex.res.layout
+        ex.release()
+        ws = rt.DeviceBuffer(lay.ws_bytes)
+        steps = mod.steps()
+        for _ in range(args.warmup):
+            rt.enqueue(mod, inputs, tun, ws, stream)
+        ev = [[rt.Event() for _ in range(len(steps) + 1)] for _ in range(args.steps)]
+        stream.sync()
+        for k in range(args.steps):
+            rt.enqueue(mod, inputs, tun, ws, stream, ev[k])
+        stream.sync()
+        totals = [ev[k][0].elapsed(ev[k][len(steps)]) for k in range(args.steps)]
+        per = {st: statistics.median(ev[k][i].elapsed(ev[k][i + 1]) for k in range(args.steps))
+               for i, st in enumerate(steps)}
+        ws.free()
+        alg, p, cfg, _ = planlib.build(name, acc)
+        plan = alg.resolveAlgebraPlan(p, cfg)
+        c0 = time.perf_counter()
+        want = evaluate(plan, acc)
+        cpu_s = time.perf_counter() - c0
+        rows = len(next(iter(want.values()))) if want else 0
+        queries[name] = {"ms": statistics.median(totals), "kernels": len(steps), "retries": retries,
+                         "result_rows": rows, "oracle_s": cpu_s, "parity_ok": sameResult(got, want),
+                         "slowest_kernel": max(per, key=per.get), "slowest_kernel_ms": max(per.values())}
+        print("[tpch] Q%d: %.3f ms (%d kernels, %d retries), %d rows, oracle %.2f s, parity %s"
+              % (q, queries[name]["ms"], len(steps), retries, rows, cpu_s, queries[name]["parity_ok"]),
+              file=sys.stderr, flush=True)
+    total_ms = sum(r["ms"] for r in queries.values())
+    li = acc.schema["lineitem"]["size"]
+    out = {"metric": METRIC, "value": 22 / (total_ms * 1e-3), "unit": "queries/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": total_ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic TPC-H (dogqc_amd/tpch.py generate, seed 10; %.1f s on the host)" % gen_s,
+           "config": {"workload": "TPC-H Q1-Q22 one after another, SF %g (%d lineitem rows), untuned plans"
+                                  % (args.sf, li), "parallelism": "single GPU"},
+           "queries": queries,
+           "parity": {"oracle": "planeval", "ok": all(r["parity_ok"] for r in queries.values())},
+           "cpu_baseline": {"value": 22 / sum(r["oracle_s"] for r in queries.values()), "unit": "queries/s",
+                            "cores": 1, "kind": "port",
+                            "sample": "the numpy plan oracle on the same 22 queries and data, once each"}}
+    print(json.dumps(out))
+
+
+def sameResult(got, want):
+    """Multiset equality of two results: rows ordered by their exact
+    (non-float) columns, then by their float columns; exact columns equal,
+    float columns within 1e-9 relative."""
+    import numpy as np
+    if set(got) != set(want):
+        return False
+    cols = list(want)
+    n = len(want[cols[0]]) if cols else 0
+    if any(len(got[c]) != n for c in cols) or any(len(want[c]) != n for c in cols):
+        return False
+    isf = {c: np.asarray(want[c]).dtype.kind == "f" for c in cols}
+    exact = [c for c in cols if not isf[c]]
+    floats = [c for c in cols if isf[c]]
+
+    def order(res):
+        keys = [[v if not isinstance(v, bytes) else v.decode("latin-1") for v in np.asarray(res[c]).tolist()]
+                for c in exact + floats]
+        return sorted(range(n), key=lambda i: tuple(k[i] for k in keys))
+    og, ow = order(got), order(want)
+    for c in cols:
+        a, b = np.asarray(got[c])[og], np.asarray(want[c])[ow]
+        if isf[c]:
+            if not np.allclose(a.astype(np.float64), b.astype(np.float64), rtol=1e-9, atol=1e-9):
+                return False
+        elif list(a.tolist()) != list(b.tolist()):
+            return False
+    return True
 
 
 def sameGroups(got, want, keyCols, tolCols):

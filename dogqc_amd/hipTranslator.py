@@ -1934,6 +1934,13 @@ class AggregationTranslator(UnaryTranslator):
         replicated R times; flushed once per workgroup."""
         pid = pipe.pipeId
         pipe.annotations.append("LDS%d" % self.algExpr.opId)
+        # an estimate beyond 8192 slots turns the tier off, except for keys of
+        # a small domain (CHAR, BOOLEAN) without a group-count hint: their
+        # groups are often few whatever the input size, so the tier stays
+        # with 4096 slots (groups beyond them go to the global table)
+        a = self.algExpr
+        bounded = _keyDomain(a.groupAttributes.values()) is not None and not getattr(a, "numgroupsFromCfg", False)
+        overflowW = "4096u" if bounded else "0"
         accBytes = sum(CType.size[acc["ctype"]] for acc in self.accs)
         # LDS in use besides the table: static arrays, a join filter in front
         used = "(uint64_t)%du" % (staticLdsBytes(pipe) + 1024)
@@ -1945,7 +1952,7 @@ class AggregationTranslator(UnaryTranslator):
             "{ int64_t w = t->lds_slots[%d]; uint32_t W = 0, R = 1;\n"
             "  if (w > 0) W = pow2floor((uint64_t)w);\n"
             "  else if (w == 0) W = std::max<uint32_t>(16u, pow2ceil((uint64_t)std::max<int64_t>(1, %s) * 2u));\n"
-            "  if (W > 8192u) W = 0;\n"
+            "  if (W > 8192u) W = %s;\n"
             "  W = std::min<uint32_t>(W, 4096u);\n"
             "  if (W) {\n"
             "    int64_t r = t->lds_replicas[%d];\n"
@@ -1955,7 +1962,7 @@ class AggregationTranslator(UnaryTranslator):
             "    if ((uint64_t)W * (8u + (uint64_t)R * %du) + %s > maxLds()) W = 0;\n"
             "  }\n"
             "  L.ldsw[%d] = W; L.ldsr[%d] = W ? R : 1u; }"
-            % (pid, est, pid, accBytes, LDS_AGG_BUDGET, accBytes, used, accBytes, used, pid, pid))
+            % (pid, est, overflowW, pid, accBytes, LDS_AGG_BUDGET, accBytes, used, accBytes, used, pid, pid))
         pipe.param("lw", "uint32_t", "L.ldsw[%d]" % pid)
         pipe.param("lr", "uint32_t", "L.ldsr[%d]" % pid)
         # dynamic LDS: tags, then one region per accumulator (8-byte aligned)

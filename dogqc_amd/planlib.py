@@ -447,6 +447,9 @@ for _c in ("c0", "c1", "c2", "c3"):
 PLANS["tpch3_profiled"] = ("tpch", {"query": 3, "buffers": [2], "profilers": [7]})
 # TPC-H Q10 as the reference's sample profiler reports ran it: a lane-activity
 # profile after the lineitem join (opId 9), without and with lane refill there
+# Q1 without its group-count hint: its four groups then run through the
+# partitioned group-by, whose partitions overflow into the global table
+PLANS["tpch1_unhinted"] = ("tpch", {"query": 1, "no_group_hints": True})
 PLANS["tpch10_p9"] = ("tpch", {"query": 10, "profilers": [9]})
 PLANS["tpch10_lr9_p9"] = ("tpch", {"query": 10, "buffers": [9], "profilers": [9]})
 # TPC-H queries with string join or group keys, their strings hashed by the
@@ -476,6 +479,10 @@ def build(name, acc):
     if kind == "tpch":
         from dogqc_amd import tpch
         alg, p, cfg = tpch.QUERIES[opts["query"]](acc)
+        if opts.get("no_group_hints"):
+            # group counts estimated from the inputs, as for a plan whose cfg
+            # gives none
+            cfg = {k: {kk: vv for kk, vv in v.items() if kk != "numgroups"} for k, v in cfg.items()}
         return alg, p, cfg, opts
     if kind == "batch":
         from dogqc_amd import batching

@@ -179,6 +179,8 @@ def q1(acc):
                 alg.map("disc_price", _disc(),
                         alg.selection(scal.SmallerEqualExpr(A("l_shipdate"), _d(19980902)),
                                       alg.scan("lineitem")))))
+    # group-count hint of the reference's cfg (query/plans/tpch1cfg.py)
+    q.hint(plan, numgroups=100)
     return alg, plan, q.cfg
 
 
@@ -240,6 +242,7 @@ def q4(acc):
                      alg.selection(scal.SmallerExpr(A("l_commitdate"), A("l_receiptdate")), alg.scan("lineitem")),
                      alg.selection(_and(scal.LargerEqualExpr(A("o_orderdate"), _d(19930701)),
                                         scal.SmallerExpr(A("o_orderdate"), _d(19931001))), alg.scan("orders"))))
+    q.hint(plan, numgroups=100)      # query/plans/tpch4cfg.py
     return alg, plan, q.cfg
 
 
@@ -331,6 +334,7 @@ def q9(acc):
     plan = alg.aggregation(["n_name", "o_year"], [(Reduction.SUM, "amount", "sum_profit")],
                            alg.map("amount", amount,
                                    alg.map("o_year", scal.ExtractExpr(A("o_orderdate"), scal.ExtractType.YEAR), lo)))
+    q.hint(plan, numgroups=1000)     # query/plans/tpch9cfg.py
     return alg, plan, q.cfg
 
 
@@ -370,6 +374,7 @@ def q12(acc):
         ["l_shipmode"], [(Reduction.SUM, "high", "high_line_count"), (Reduction.SUM, "low", "low_line_count")],
         alg.map("low", scal.CaseExpr([(high, _i(0))], _i(1)),
                 alg.map("high", scal.CaseExpr([(high, _i(1))], _i(0)), ol)))
+    q.hint(plan, numgroups=7)        # query/plans/tpch12cfg.py
     return alg, plan, q.cfg
 
 

@@ -535,7 +535,7 @@ def test_batched_plan_on_gpu(config):
 # plans with a group-by the translator can launch partitioned (exact keys, a
 # pipeline loop uniform over the workgroup)
 PART_FEATURES = ["f_semi", "f_anti_multi", "f_anti_neg", "f_cross", "f_like_patterns"]
-PART_TPCH = [1, 2, 8, 11, 15, 18, 20]
+PART_TPCH = ["1_unhinted", 2, 8, 11, 15, 18, 20]
 
 
 def partTuning(parts):
@@ -564,10 +564,11 @@ def test_partitioned_groupby_features(name, parts):
 @pytest.mark.parametrize("parts", [2, 64, 1024])
 @pytest.mark.parametrize("q", PART_TPCH)
 def test_partitioned_groupby_tpch(q, parts):
-    """TPC-H group-bys through the partitioned tier.  Q1's 4 groups land in at
-    most 4 partitions, far more rows than a partition holds: the rows beyond
-    its capacity update the global table directly."""
-    name = "tpch%d" % q
+    """TPC-H group-bys through the partitioned tier.  Q1's 4 groups (its plan
+    without the group-count hint) land in at most 4 partitions, far more rows
+    than a partition holds: the rows beyond its capacity update the global
+    table directly."""
+    name = "tpch%s" % q
     assertPartitioned(name)
     check(name, tpchAcc(0.05), tuning=partTuning(parts), storeKey=("tpch", 0.05), label="parts=%d" % parts)
 
