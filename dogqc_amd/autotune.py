@@ -8,7 +8,8 @@ relationalAlgebra.py:453-455) plus the MI355X-specific ones:
   compile time (one module per variant, built ahead of time):
     workgroup size, rows per thread
   run time (dqc_plan_tuning, no recompilation):
-    grid size per pipeline, refill threshold, LDS aggregation slots and
+    grid size per pipeline, refill threshold, long-list split threshold
+    (pipelines annotated SPLITn), LDS aggregation slots and
     replicas (pipelines annotated LDSn), hash-table size factor, and the
     size of the join filter (LDS sizes together with the probe pipeline's
     grid, MB sizes kept in device memory)
@@ -184,6 +185,11 @@ class Autotuner(object):
             trySet("grid", pid, [cus * k for k in (1, 2, 4, 8, 16, 32)], pipe)
             if any(a.startswith("LR") for a in pipe["annotations"]):
                 trySet("refill_threshold", pid, [8, 16, 32, 48, 64], pipe)
+            if any(a.startswith("SPLIT") for a in pipe["annotations"]):
+                # list length above which a probe row's matches go to the
+                # second launch (judged by the whole plan: the split scan and
+                # both launches move with it)
+                trySet("split_threshold", pid, [128, 256, 512, 2048], None)
             if any(a.startswith("LDS") for a in pipe["annotations"]):
                 # LDS aggregation width: table slots (-1: no LDS tier), then
                 # accumulator replicas

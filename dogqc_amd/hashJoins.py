@@ -73,7 +73,10 @@ class EquiJoinTranslator(BinaryTranslator):
     # unique tables with 32-bit tags and one stored column of at most 4 bytes
     # keep tag and column in one 64-bit slot (dqc::packed_slot)
     usePackedSlots = True
-    SPLIT_CHUNK = 64                 # matches per queued chunk
+    # matches per queued chunk: a lane of the second launch takes one chunk,
+    # so smaller chunks give that launch more wavefronts (c2 at 16 M probe
+    # rows, second launch: 64 -> 76 us, 32 -> 47 us, 16 -> 33 us, 8 -> 31 us)
+    SPLIT_CHUNK = 16
     DEFAULT_SPLIT_THRESHOLD = 1024   # list length above which a probe row is queued
     # unique inner joins at most this selective can take a filter of their
     # build keys (tuning join_filter = n KB, up to 4 MB; off by default): in
@@ -1031,7 +1034,7 @@ class EquiJoinTranslator(BinaryTranslator):
         pipe.code.close()
         pipe.code.close()
         pipe.phased = True
-        pipe.phaseRows[1] = ("(int64_t)%s * 64" % cap, "*%s" % ftot)
+        pipe.phaseRows[1] = ("(int64_t)%s * %d" % (cap, max(64, 4096 // CH)), "*%s" % ftot)
 
         def scan(prog, idx=idx, blen=blen, bcs=bcs, btot=btot, cap=cap, CH=CH):
             return ["hipLaunchKernelGGL(dqc::dqc_split_scan, dim3(1), dim3(dqc::SCAN_BS), 0, s, "
