@@ -1116,12 +1116,26 @@ __global__ void __launch_bounds__(SCAN_BS) dqc_split_scan(const uint32_t* len, c
     __shared__ uint64_t sh[SCAN_BS / WAVE + 1];
     const uint64_t n = *count < cap ? *count : cap;
     uint64_t carry = 0;
-    for (uint64_t base = 0; base < n; base += SCAN_BS) {
-        const uint64_t ix = base + threadIdx.x;
-        const uint64_t v = ix < n ? ((uint64_t)len[ix] + chunk - 1u) / chunk : 0u;
+    // SCAN_ITEMS consecutive items per thread and trip: one block scan per
+    // SCAN_TILE items
+    for (uint64_t base = 0; base < n; base += SCAN_TILE) {
+        const uint64_t b0 = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+        uint64_t v[SCAN_ITEMS];
+        uint64_t sum = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i) {
+            const uint64_t ix = b0 + (uint64_t)i;
+            v[i] = ix < n ? ((uint64_t)len[ix] + chunk - 1u) / chunk : 0u;
+            sum += v[i];
+        }
         uint64_t t;
-        const uint64_t ex = block_excl_scan(v, sh, t);
-        if (ix < n) cstart[ix] = carry + ex;
+        uint64_t run = carry + block_excl_scan(sum, sh, t);
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; ++i) {
+            const uint64_t ix = b0 + (uint64_t)i;
+            if (ix < n) cstart[ix] = run;
+            run += v[i];
+        }
         carry += t;
     }
     if (threadIdx.x == 0) {
