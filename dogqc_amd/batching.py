@@ -102,7 +102,7 @@ def plan(config, acc):
         cfg = {j1.opId: {"multimatch": False, "selectivity": 0.06}, j2.opId: {"multimatch": False},
                j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 176 * MEMBERS},
                byItem.opId: {"partitions": 1024}}
-        return alg, [star, ny, byItem], cfg, {"buffers": [j1.opId], "nontemporal": "all"}
+        return alg, [star, ny, byItem], cfg, {"buffers": [j1.opId], "nontemporal": True}
     raise KeyError(config)
 
 

@@ -1045,8 +1045,7 @@ class ScanTranslator(LiteralTranslator):
     EAGER_LINE_FRACTION = 0.8
     # base-table column loads with the nontemporal hint (streamed through
     # the caches with low priority, so hash tables probed beside the scan
-    # keep their lines): False, "eager" (the columns loaded at the top of
-    # the pipeline, whose lines the wavefront reads whole) or "all"
+    # keep their lines)
     nontemporal = False
 
     def produce(self, ctxt):
@@ -1063,7 +1062,6 @@ class ScanTranslator(LiteralTranslator):
         pipe.scanTable = tname
         ctxt.pipe = pipe
         tid = pipe.tid + "[u]"
-        self.ntLoaders = {}
         for id, attr in a.outRelation.items():
             if attr.dataType == Type.STRING:
                 oi = prog.column(tname, attr.name, "offset", CType.UINT64)
@@ -1077,9 +1075,10 @@ class ScanTranslator(LiteralTranslator):
                 ci = prog.column(tname, attr.name, None, ct)
                 f = pipe.param("c%d" % ci, "const %s*" % ct, "C_%d" % ci)
                 lt = langType(attr.dataType, prog.decimalRepr)
-                plain, nt = "(%s)%s[%s]" % (lt, f, tid), "(%s)__builtin_nontemporal_load(&%s[%s])" % (lt, f, tid)
-                self.ntLoaders[attr.id] = nt
-                pipe.setLazy(attr, nt if ScanTranslator.nontemporal == "all" else plain, [pipe.tid])
+                ref = "%s[%s]" % (f, tid)
+                if ScanTranslator.nontemporal:
+                    ref = "__builtin_nontemporal_load(&%s)" % ref
+                pipe.setLazy(attr, "(%s)%s" % (lt, ref), [pipe.tid])
         pipe.param("st", "dqc_plan_status*", "st")
         self._finish(ctxt, pipe)
 
@@ -1093,10 +1092,6 @@ class ScanTranslator(LiteralTranslator):
             if lineTouchFraction(fracs[id], size) >= ScanTranslator.EAGER_LINE_FRACTION:
                 eager.append(attr)
         if eager:
-            if ScanTranslator.nontemporal == "eager":
-                for attr in eager:
-                    if attr.id in getattr(self, "ntLoaders", {}):
-                        pipe.attrs[attr.id].loader = self.ntLoaders[attr.id]
             pipe.need(eager)
         self.parent.consume(ctxt)
         closePipeline(ctxt, pipe)

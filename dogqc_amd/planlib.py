@@ -407,7 +407,7 @@ PLANS["c1_refill"] = ("c1", {"buffers": [4]})
 # 0.500 ms); c1 and c2 ran slower with it (c1_refill's fact kernel 0.031 ->
 # 0.035 ms, c2's probe 0.217 -> 0.229 ms), and hinting only the loads at the
 # top of a pipeline gained nothing on c3
-C3 = {"nontemporal": "all"}
+C3 = {"nontemporal": True}
 PLANS["c3"] = ("c3", dict(C3))
 PLANS["c3_refill"] = ("c3", dict(C3, buffers=[6]))
 PLANS["cal"] = ("cal", {})
