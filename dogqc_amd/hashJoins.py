@@ -85,6 +85,9 @@ class EquiJoinTranslator(BinaryTranslator):
     # searches size and grid together), else probed in device memory, where
     # a filter of a few MB stays in L2 while the table it guards does not
     FILTER_SELECTIVITY = 0.25
+    # join filter size (KB) a plan uses when its tuning leaves join_filter at
+    # 0; plans translated with it carry it in their generated host code
+    defaultJoinFilter = 0
 
     def __init__(self, algExpr, leftChild, rightChild):
         BinaryTranslator.__init__(self, algExpr, leftChild, rightChild)
@@ -250,8 +253,12 @@ class EquiJoinTranslator(BinaryTranslator):
             fields = [StoredField(fe, k) for k in range(len(self.stored))]
         fbf = None
         if kind == "unique" and self.filtered:
+            # tuning join_filter: n KB, < 0 none, 0 the plan's default
+            # (defaultJoinFilter, none unless a plan is translated with one)
+            req = "t->join_filter" if not EquiJoinTranslator.defaultJoinFilter else \
+                "(t->join_filter != 0 ? t->join_filter : %d)" % int(EquiJoinTranslator.defaultJoinFilter)
             prog.hostVars["bfw%d" % idx] = (
-                "{ const int32_t req = t->join_filter; uint32_t w = 0u;\n"
+                "{ const int32_t req = " + req + "; uint32_t w = 0u;\n"
                 "  if (req > 0) w = std::min<uint32_t>(1u << 20, pow2floor((uint64_t)req * 256u));\n"
                 "  L.bfw[%d] = w; }" % idx)
             bf = prog.buffer("jbf%d" % J, CType.UINT, "std::max<uint32_t>(1u, L.bfw[%d])" % idx, "0x0ull")

@@ -25,11 +25,12 @@ from dogqc_amd.kernel import KernelCall
 
 @contextlib.contextmanager
 def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBuild=False, bufferType=None,
-          waveStrings=False, nontemporal=False):
+          waveStrings=False, nontemporal=False, joinFilter=0):
     old = (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
            KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType,
-           StringHash.wave, ScanTranslator.nontemporal)
+           StringHash.wave, ScanTranslator.nontemporal, EquiJoinTranslator.defaultJoinFilter)
     ScanTranslator.nontemporal = nontemporal
+    EquiJoinTranslator.defaultJoinFilter = joinFilter
     EquiJoinTranslator.usePushDownJoin = pushdown
     EquiJoinTranslator.usePartitionedBuild = partBuild
     LaneRefill.bufferType = bufferType or BufferType.SMEM
@@ -45,7 +46,7 @@ def knobs(pushdown=False, block=None, threshold=None, rowsPerThread=None, partBu
     finally:
         (EquiJoinTranslator.usePushDownJoin, KernelCall.defaultBlockSize, LaneRefill.threshold,
          KernelCall.defaultRowsPerThread, EquiJoinTranslator.usePartitionedBuild, LaneRefill.bufferType,
-         StringHash.wave, ScanTranslator.nontemporal) = old
+         StringHash.wave, ScanTranslator.nontemporal, EquiJoinTranslator.defaultJoinFilter) = old
 
 
 # ------------------------------------------------------------- feature plans
@@ -535,7 +536,8 @@ def translate(name, acc=None, decimalRepr=CType.FP64, block=None, rowsPerThread=
                bufferType={"reg": BufferType.REG, "global": BufferType.GLOBAL}.get(opts.get("buffer_type"),
                                                                                     BufferType.SMEM),
                waveStrings=opts.get("wave_strings", False),
-               nontemporal=opts.get("nontemporal", False)):
+               nontemporal=opts.get("nontemporal", False),
+               joinFilter=opts.get("join_filter", 0)):
         comp = HipCompiler(alg, "gfx950", decimalRepr)
         comp.setBuffers(opts.get("buffers", []))
         comp.setProfilers(opts.get("profilers", []))
