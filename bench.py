@@ -438,10 +438,12 @@ def graphReplay(mod, inputs, tuning, ws, stream, steps):
 
 def benchTpch(args):
     """TPC-H Q1-Q22 on synthetic data at --sf, each plan as `dogqc_amd.tpch`
-    writes it (untuned): per query the median time of --steps enqueues
-    (after --warmup), its kernels, retries of the first run, and its result
-    against the numpy plan oracle (whose time is the CPU column)."""
-    from dogqc_amd import planlib, runtime as rt, runner, tpch
+    writes it, with the autotuner's record of its size bucket when there is
+    a current one (--untuned: never): per query the median time of --steps
+    enqueues (after --warmup), its kernels, retries of the first run, and
+    its result against the numpy plan oracle (whose time is the CPU
+    column)."""
+    from dogqc_amd import autotune, planlib, runtime as rt, runner, tpch
     from dogqc_amd.dbio import MemoryAccessor
     from oracle.planeval import evaluate
     t0 = time.perf_counter()
@@ -450,11 +452,20 @@ def benchTpch(args):
     store = {}
     stream = rt.Stream()
     queries = {}
+    li = acc.schema["lineitem"]["size"]
     for q in range(1, 23):
         name = "tpch%d" % q
-        mod = rt.PlanModule(planlib.translate(name).compile(name))
+        # the record of the plan's own size bucket: its largest input table
+        plain = planlib.translate(name)
+        driving = max(acc.schema[t]["size"] for t in plain.program.tables)
+        rec = None if args.untuned else autotune.nearest(autotune.records(name), driving)
+        if rec is not None:
+            mod = rt.PlanModule(autotune.variantPath(name, rec["block"], rec["rows_per_thread"]))
+            tun = autotune.tuningFromDict(rec["tuning"])
+        else:
+            mod = rt.PlanModule(plain.compile(name))
+            tun = rt.newTuning()
         inputs = rt.BoundInputs(mod, acc, store=store)
-        tun = rt.newTuning()
         ex = rt.run(mod, inputs, tun, stream)       # retries grow tun's capacities in place
         retries = int(ex.res.retries)
         got = runner.resultColumns(mod, ex, acc)
@@ -480,19 +491,21 @@ def benchTpch(args):
         cpu_s = time.perf_counter() - c0
         rows = len(next(iter(want.values()))) if want else 0
         queries[name] = {"ms": statistics.median(totals), "kernels": len(steps), "retries": retries,
+                         "autotuned": None if rec is None else {"block": rec["block"],
+                                                                "rows_per_thread": rec["rows_per_thread"]},
                          "result_rows": rows, "oracle_s": cpu_s, "parity_ok": sameResult(got, want),
                          "slowest_kernel": max(per, key=per.get), "slowest_kernel_ms": max(per.values())}
         print("[tpch] Q%d: %.3f ms (%d kernels, %d retries), %d rows, oracle %.2f s, parity %s"
               % (q, queries[name]["ms"], len(steps), retries, rows, cpu_s, queries[name]["parity_ok"]),
               file=sys.stderr, flush=True)
     total_ms = sum(r["ms"] for r in queries.values())
-    li = acc.schema["lineitem"]["size"]
     out = {"metric": METRIC, "value": 22 / (total_ms * 1e-3), "unit": "queries/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": total_ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
            "data": "synthetic TPC-H (dogqc_amd/tpch.py generate, seed 10; %.1f s on the host)" % gen_s,
-           "config": {"workload": "TPC-H Q1-Q22 one after another, SF %g (%d lineitem rows), untuned plans"
-                                  % (args.sf, li), "parallelism": "single GPU"},
+           "config": {"workload": "TPC-H Q1-Q22 one after another, SF %g (%d lineitem rows), %d of them with "
+                                  "autotuner records" % (args.sf, li, sum(1 for r in queries.values() if r["autotuned"])),
+                      "parallelism": "single GPU"},
            "queries": queries,
            "parity": {"oracle": "planeval", "ok": all(r["parity_ok"] for r in queries.values())},
            "cpu_baseline": {"value": 22 / sum(r["oracle_s"] for r in queries.values()), "unit": "queries/s",

@@ -44,6 +44,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CACHE_DIR = os.path.join(PKG, "_tuning")
 
 VARIANTS = [(256, 4), (256, 2), (256, 1), (512, 4), (128, 4), (256, 8), (512, 1), (1024, 1)]
+# the TPC-H plans are tuned over fewer compile variants (22 plans)
+TPCH_VARIANTS = [(256, 4), (512, 1), (1024, 1)]
 
 
 # dqc_plan_tuning fields a retry grows (dqc_runtime.hip adjustForRetry)
@@ -385,25 +387,39 @@ def main(argv=None):
         for v in variants:
             if v not in VARIANTS:
                 ap.error("variant %dx%d is not built (built: %s)" % (v + (", ".join("%dx%d" % w for w in VARIANTS),)))
+        if any(planlib.PLANS[n][0] == "tpch" for n in args.plan) and any(v not in TPCH_VARIANTS for v in variants):
+            ap.error("TPC-H plans are built in the variants %s" % ", ".join("%dx%d" % w for w in TPCH_VARIANTS))
     if args.serving:
         tuneServing(args.plan, [float(x) for x in args.scales.split(",")], reps=args.reps, out=args.out_dir,
                     log=lambda m: print(m, flush=True), variants=variants)
         return
-    args.plan = args.plan[0]
-    kind = planlib.PLANS[args.plan][0]
-    rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[kind].items()}
-    acc = MemoryAccessor(workloads.generate(kind, rows))
-    tuner = Autotuner(args.plan, acc, variants=variants, reps=args.reps, verbose=True)
-    best, allRecs = tuner.run()
-    best["scale"] = args.rows_scale
-    if args.out_dir:
-        saveRecord(best, args.out_dir)
-    else:
-        saveCached(args.plan, acc, best)
-    if args.out:
-        with open(args.out, "w") as f:
-            json.dump({"best": best, "all": allRecs}, f, indent=1, sort_keys=True)
-    print(json.dumps(best))
+    kinds = set(planlib.PLANS[n][0] for n in args.plan)
+    if kinds == {"tpch"}:
+        # TPC-H plans: synthetic data at scale factor --rows-scale, once for
+        # all of them, and their own compile variants
+        from dogqc_amd import tpch
+        tacc = MemoryAccessor(tpch.generate(args.rows_scale, seed=10))
+        variants = variants or TPCH_VARIANTS
+    elif len(args.plan) > 1:
+        ap.error("several plans at once only for TPC-H plans")
+    for name in args.plan:
+        kind = planlib.PLANS[name][0]
+        if kind == "tpch":
+            acc = tacc
+        else:
+            rows = {t: max(1, int(n * args.rows_scale)) for t, n in workloads.FULL_ROWS[kind].items()}
+            acc = MemoryAccessor(workloads.generate(kind, rows))
+        tuner = Autotuner(name, acc, variants=variants, reps=args.reps, verbose=True)
+        best, allRecs = tuner.run()
+        best["scale"] = args.rows_scale
+        if args.out_dir:
+            saveRecord(best, args.out_dir)
+        else:
+            saveCached(name, acc, best)
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump({"best": best, "all": allRecs}, f, indent=1, sort_keys=True)
+        print(json.dumps(best), flush=True)
 
 
 if __name__ == "__main__":

@@ -86,6 +86,11 @@ def buildVariants(jobs=None):
         for block, rows in autotune.VARIANTS:
             comp = autotune.translateVariant(name, block, rows)
             work.append((comp, autotune.variantName(name, block, rows)))
+    for q in range(1, 23):
+        for block, rows in autotune.TPCH_VARIANTS:
+            name = "tpch%d" % q
+            comp = autotune.translateVariant(name, block, rows)
+            work.append((comp, autotune.variantName(name, block, rows)))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         return list(ex.map(lambda cn: cn[0].compile(cn[1]), work))
