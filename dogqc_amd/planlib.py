@@ -428,6 +428,12 @@ PLANS["f_like_project_refill_reg"] = ("feature", {"buffers": [2], "base": "f_lik
 # TPC-H Q1-Q22 (dogqc_amd/tpch.py)
 for _q in range(1, 23):
     PLANS["tpch%d" % _q] = ("tpch", {"query": _q})
+# Q10 refills lanes after the lineitem join (opId 9), the placement of
+# the reference's balanced Q10 (sample/tpch10_balanced.cu:356-457). Tuned at
+# SF1 over the three TPC-H compile variants on MI355X: 0.565 ms without a
+# refill, 0.237 ms with the LDS queue here, 0.242 ms with the queue across
+# two launches (BufferType.GLOBAL).
+PLANS["tpch10"] = ("tpch", {"query": 10, "buffers": [9]})
 
 # plans translated with the partitioned unique build (tuning part_build)
 PB_FEATURES = ["f_groupjoin", "f_join_unique_lowest", "f_semi", "f_temp", "f_join_neg", "f_anti_neg", "f_join_packed"]
