@@ -426,14 +426,23 @@ PLANS["f_like_project_refill_reg"] = ("feature", {"buffers": [2], "base": "f_lik
 
 
 # TPC-H Q1-Q22 (dogqc_amd/tpch.py)
+# Lane refill (LDS queue) at one operator of some queries. Q10 refills after
+# its lineitem join (opId 9), the placement of the reference's balanced Q10
+# (sample/tpch10_balanced.cu:356-457): tuned at SF1 on MI355X, 0.565 ms
+# without it, 0.237 ms with it (0.242 ms with the queue across two launches).
+# The others come from timing a refill at every selection and join with
+# >= 100 K input rows of every query, SF1, 512x1, untuned, results equal to
+# the plan without it, and keeping those 8% or more faster: Q3 0.165 ->
+# 0.143 ms, Q5 0.200 -> 0.183, Q7 0.197 -> 0.179, Q8 0.204 -> 0.174,
+# Q9 0.389 -> 0.342, Q14 0.081 -> 0.073, Q19 0.240 -> 0.179.
+TPCH_REFILL = {3: 8, 5: 13, 7: 13, 8: 13, 9: 6, 10: 9, 14: 3, 19: 2}
 for _q in range(1, 23):
-    PLANS["tpch%d" % _q] = ("tpch", {"query": _q})
-# Q10 refills lanes after the lineitem join (opId 9), the placement of
-# the reference's balanced Q10 (sample/tpch10_balanced.cu:356-457). Tuned at
-# SF1 over the three TPC-H compile variants on MI355X: 0.565 ms without a
-# refill, 0.237 ms with the LDS queue here, 0.242 ms with the queue across
-# two launches (BufferType.GLOBAL).
-PLANS["tpch10"] = ("tpch", {"query": 10, "buffers": [9]})
+    PLANS["tpch%d" % _q] = ("tpch", {"query": _q, "buffers": [TPCH_REFILL[_q]]} if _q in TPCH_REFILL
+                            else {"query": _q})
+# the same queries without the refill (a pipeline with a refill point keeps
+# its group-by out of the partitioned tier)
+for _q in TPCH_REFILL:
+    PLANS["tpch%d_plain" % _q] = ("tpch", {"query": _q})
 
 # plans translated with the partitioned unique build (tuning part_build)
 PB_FEATURES = ["f_groupjoin", "f_join_unique_lowest", "f_semi", "f_temp", "f_join_neg", "f_anti_neg", "f_join_packed"]

@@ -535,7 +535,7 @@ def test_batched_plan_on_gpu(config):
 # plans with a group-by the translator can launch partitioned (exact keys, a
 # pipeline loop uniform over the workgroup)
 PART_FEATURES = ["f_semi", "f_anti_multi", "f_anti_neg", "f_cross", "f_like_patterns"]
-PART_TPCH = ["1_unhinted", 2, 8, 11, 15, 18, 20]
+PART_TPCH = ["1_unhinted", 2, "8_plain", 11, 15, 18, 20]
 
 
 def partTuning(parts):
