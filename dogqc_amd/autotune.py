@@ -198,6 +198,19 @@ class Autotuner(object):
                 trySet("lds_slots", pid, [-1, 256, 1024, 4096], pipe)
                 if tuning.lds_slots[pid] >= 0:
                     trySet("lds_replicas", pid, [1, 4, 16, 64], pipe)
+        if any(a.startswith("PART") for p in self._pipelines(module) for a in p["annotations"]):
+            # partitions of the two-phase group-by (0: from the group count,
+            # -1: one phase), judged by the whole plan
+            keep, bestScore = tuning.part_agg, score(best)
+            for v in (-1, 16, 64, 256, 1024):
+                if v == keep:
+                    continue
+                tuning.part_agg = v
+                times, fits = m.measure(tuning)
+                self.log.append(("part_agg", v, score(times)))
+                if fits and score(times) < bestScore * 0.98:
+                    keep, bestScore, best = v, score(times), times
+            tuning.part_agg = keep
         for ht in module.desc["hts"]:
             if tuning.ht_capacity[ht["index"]] > 0:
                 continue        # an explicit capacity overrides the factor
